@@ -1,0 +1,159 @@
+"""Benchmark: field evaluation of a 2^22-particle relativistic Gaussian bunch on
+one MI355X, mixed-precision FMM vs the FP64 HIP FMM.
+
+A step is one full update_particles_field evaluation (mean momentum, tree,
+lists, P2M/M2M, M2L, P2P, L2L/L2P) with positions and momenta already
+resident in HBM and the fields written to HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4] [--N 4194304]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fmm4rbgpu.jl_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+METRIC = "particles/s field eval N=2^22: mixed-prec vs FP64 HIP FMM at rel err <=1e-5"
+PEAK = {"mixed": 157.3e12, "fp64": 78.6e12}  # MI355X FP32 / FP64 vector peaks (FLOP/s)
+FLOPS_PER_PAIR = 40  # one kernel evaluation, P2P particle pair or M2L node pair
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--N", type=int, default=1 << 22)
+    ap.add_argument("--n", type=int, default=4)
+    ap.add_argument("--N0", type=int, default=125)
+    ap.add_argument("--eta", type=float, default=0.5)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 17,
+                    help="particles of the serial CPU FMM sample (0 = skip)")
+    ap.add_argument("--skip-fp64", action="store_true")
+    return ap.parse_args()
+
+
+def timed(ctx, torch, N, bufs, n, N0, eta, prec, steps, warmup, lam):
+    dpos, dmom, dE, dB = bufs
+    call = lambda: ctx.field_device(N, dpos.data_ptr(), dmom.data_ptr(), dE.data_ptr(),
+                                    dB.data_ptr(), -1.0, lam, n, N0, eta, prec)
+    for _ in range(warmup):
+        call()
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(call())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall / steps, stats
+
+
+def stage_avg(stats):
+    keys = ["ms_total", "ms_pavg", "ms_tree", "ms_upward", "ms_lists", "ms_m2l", "ms_p2p",
+            "ms_downward"]
+    return {k[3:]: sum(getattr(s, k) for s in stats) / len(stats) for k in keys}
+
+
+def roofline(st, stages, n, prec):
+    nn = (n + 1) ** 3
+    m2l_flops = FLOPS_PER_PAIR * st.nm2l * nn * nn
+    p2p_flops = FLOPS_PER_PAIR * st.p2p_particle_pairs
+    kern = {"m2l": (m2l_flops, stages["m2l"]), "p2p": (p2p_flops, stages["p2p"])}
+    name = max(kern, key=lambda k: kern[k][1])
+    flops, ms = kern[name]
+    ach = flops / (ms * 1e-3)
+    other = "p2p" if name == "m2l" else "m2l"
+    of, oms = kern[other]
+    return {"kernel": name, "bound": "valu", "achieved": ach / 1e12, "peak": PEAK[prec] / 1e12,
+            "unit": "TFLOP/s", "frac": ach / PEAK[prec], "traffic": None,
+            "flops_per_launch": flops, "avg_launch_ms": ms,
+            "other": {"kernel": other, "achieved": of / (oms * 1e-3) / 1e12,
+                      "frac": of / (oms * 1e-3) / PEAK[prec], "avg_launch_ms": oms}}
+
+
+def cpu_baseline(N, n, N0, eta):
+    import oracle
+    from fmm4rb_hip import workloads as W
+    pos, mom = W.bunch(N, seed=2)
+    t0 = time.perf_counter()
+    oracle.fmm(pos, mom, -1.0, 1e-3, n, N0, eta)
+    dt = time.perf_counter() - t0
+    return {"value": N / dt, "unit": "particles/s", "cores": 1, "kind": "port",
+            "sample": f"serial FMM (C restatement of src/fmm, 1 thread) on a {N}-particle bunch "
+                      f"of the same distribution, n={n} N0={N0} eta={eta}: {dt:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the multi-GPU sharding design is documented, not built)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import fmm4rb_hip as F
+    from fmm4rb_hip import workloads as W
+
+    lam = 1e-3
+    dev = torch.device("cuda:0")
+    pos, mom = W.bunch(a.N, seed=3)
+    dpos = torch.from_numpy(pos).to(dev)
+    dmom = torch.from_numpy(mom).to(dev)
+    bufs_m = (dpos, dmom, torch.empty_like(dpos), torch.empty_like(dpos))
+    bufs_d = (dpos, dmom, torch.empty_like(dpos), torch.empty_like(dpos))
+    torch.cuda.synchronize()
+    ctx = F.Context(0)
+    ms_m, st_m = timed(ctx, torch, a.N, bufs_m, a.n, a.N0, a.eta, "mixed", a.steps, a.warmup, lam)
+    res = {}
+    if not a.skip_fp64:
+        ms_d, st_d = timed(ctx, torch, a.N, bufs_d, a.n, a.N0, a.eta, "fp64", a.steps, a.warmup,
+                           lam)
+        res["fp64"] = {"value": a.N / ms_d, "ms_per_step": ms_d * 1e3,
+                       "stages_ms": stage_avg(st_d),
+                       "roofline": roofline(st_d[-1], stage_avg(st_d), a.n, "fp64")}
+    # accuracy, outside the timed region
+    Em, Bm = bufs_m[2].cpu().numpy(), bufs_m[3].cpu().numpy()
+    acc = {}
+    if not a.skip_fp64:
+        Ed, Bd = bufs_d[2].cpu().numpy(), bufs_d[3].cpu().numpy()
+        acc["mixed_vs_fp64"] = {"E": W.rel_l2(Em, Ed), "B": W.rel_l2(Bm, Bd)}
+    tg = np.random.default_rng(40).choice(a.N, 4096, replace=False)
+    Eb, Bb = ctx.bruteforce(pos, mom, -1.0, lam, targets=tg)
+    acc["mixed_vs_bruteforce_4096"] = {"E": W.rel_l2(Em[tg], Eb), "B": W.rel_l2(Bm[tg], Bb)}
+    if not a.skip_fp64:
+        acc["fp64_vs_bruteforce_4096"] = {"E": W.rel_l2(Ed[tg], Eb), "B": W.rel_l2(Bd[tg], Bb)}
+    stages = stage_avg(st_m)
+    s = st_m[-1]
+    out = {
+        "metric": METRIC, "value": a.N / ms_m, "unit": "particles/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_m * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64 (mixed)", "data": "synthetic",
+        "config": {"workload": f"relativistic Gaussian bunch, N={a.N}, sigma=(1e-3,1e-3,1e-4), "
+                               "p_z~N(100,1), p_x,p_y~N(0,0.01), lambda=1e-3",
+                   "N": a.N, "n": a.n, "N0": a.N0, "eta": a.eta, "precision": "mixed"},
+        "accuracy": acc,
+        "rel_err_ok": (acc.get("mixed_vs_fp64", {"E": 0, "B": 0})["E"] <= 1e-5
+                       and acc.get("mixed_vs_fp64", {"E": 0, "B": 0})["B"] <= 1e-5),
+        "counts": {"ncluster": s.ncluster, "nleaf": s.nleaf, "np2p": s.np2p, "nm2l": s.nm2l,
+                   "p2p_particle_pairs": s.p2p_particle_pairs, "maxlevel": s.maxlevel},
+        "stages_ms": stages,
+        "roofline": roofline(s, stages, a.n, "mixed"),
+    }
+    if "fp64" in res:
+        out["fp64"] = res["fp64"]
+        out["speedup_vs_fp64"] = out["value"] / res["fp64"]["value"]
+    ctx.close()
+    if a.cpu_sample > 0:
+        out["cpu_baseline"] = cpu_baseline(a.cpu_sample, a.n, a.N0, a.eta)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

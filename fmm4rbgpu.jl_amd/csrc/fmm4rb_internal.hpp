@@ -1,0 +1,146 @@
+// fmm4rb_internal.hpp -- shared declarations of the gfx950 FMM library.
+//
+// Data layout in HBM (N particles, nc clusters, NP = n+1, NN = NP^3):
+//   sp8   [N][8]  double  particles in tree order: x, y, z, gamma, px, py, pz, 0
+//   sp8f  [N][8]  float   same, positions relative to the particle's leaf centre
+//   geo   [nc][8] double  centre xyz, admissibility radius, diameter, 0, 0, 0
+//   ncoord[nc][3][NP]     Chebyshev (2nd kind) interpolation nodes per axis
+//   mp4   [nc][NN] double4 multipoles (gamma, mx, my, mz)
+//   mp4f  [nc][NN] float4  FP32 copy for the mixed-precision M2L
+//   loc   [nc][NN][6]     local expansions (Ex, Ey, Ez, Bx, By, Bz)
+//   near  [N][6]          P2P sums in tree order
+// Nodes are 0-based heap ids (children 2k+1, 2k+2).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fmm4rb.h"
+
+namespace fmm4rb {
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define FMM_HIP(x)                                                                                \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess)                                                                     \
+            throw ::fmm4rb::Error{FMM4RB_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define FMM_LAUNCH_CHECK() FMM_HIP(hipGetLastError())
+
+inline void fail(int code, const std::string &m) { throw Error{code, m}; }
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    void ensure(size_t bytes);
+    void release();
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+constexpr double EPS64 = 2.220446049250313e-16;  // Julia eps()
+constexpr float EPS32K = 2.220446049250313e-16f;  // the same softening in FP32
+constexpr double AMP_RE = 2.8179403699772166e-15;
+constexpr int MAX_NP = 11;      // n <= 10
+constexpr int MAX_N0 = 1024;
+
+// Tree shape: a closed-form function of (N, N0) (clusters.jl:21-49 splits a
+// node of c particles into ceil(c/2) and floor(c/2) while c > N0).
+struct Shape {
+    int64_t N = -1, N0 = -1;
+    int ml = 0;
+    int64_t nc = 0;
+    std::vector<int32_t> lo, cnt;       // per node (-1 / 0 if absent)
+    std::vector<uint8_t> flags;         // bit0 exists, bit1 leaf
+    std::vector<int32_t> leaves;        // existing leaves, ascending id
+    std::vector<int32_t> lvl_nodes;     // existing nodes grouped by level
+    std::vector<int64_t> lvl_off;       // ml+2 offsets into lvl_nodes
+    std::vector<int32_t> lvl_split;     // existing internal nodes grouped by level
+    std::vector<int64_t> split_off;     // ml+2 offsets into lvl_split
+    std::vector<int32_t> leftbase;      // per splitting node: lefts before it on its level
+    int maxleaf = 0;
+};
+
+struct Ctx;
+
+// ---- radix sort (radix_sort.hip)
+// Sort nseg independent arrays of n (key, value) pairs by key bits [bit0, bit1).
+// Returns true if the result is in the *_alt buffers.
+size_t radix_scratch_words(int64_t n, int nseg);
+bool radix_sort(uint64_t *keys, uint64_t *keys_alt, uint32_t *vals, uint32_t *vals_alt,
+                int64_t n, int nseg, int bit0, int bit1, uint32_t *scratch, hipStream_t st);
+// Exclusive scan of nrows rows of len words each; row totals to tot[nrows].
+void scan_rows(uint32_t *rows, int nrows, int len, uint32_t *tot, hipStream_t st);
+
+// ---- tree (tree.hip)
+void launch_pavg(const double *pos, const double *mom, int64_t N, double *part, double *pavg4,
+                 int *bad, hipStream_t st);
+void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, int64_t N0);
+
+// ---- traversal (traverse.hip)
+void build_lists_device(Ctx &c, double eta);
+void sort_pairs_csr(Ctx &c, uint64_t *keys, uint64_t *keys_alt, int64_t n, int32_t *off,
+                    uint64_t **sorted);
+
+// ---- interpolation operators (interp.hip)
+void launch_node_geo(Ctx &c);
+void launch_upward(Ctx &c);
+void launch_downward(Ctx &c, double amp, double *E, double *B);
+void launch_l2l(Ctx &c);
+void launch_l2p(Ctx &c, double amp, double *E, double *B);
+void launch_mp_to_f32(Ctx &c);
+
+// ---- interactions (m2l.hip, p2p.hip)
+void launch_m2l(Ctx &c, const int32_t *off, const uint64_t *keys, int sb, int64_t ngroups_grid);
+void launch_p2p(Ctx &c, const int32_t *off, const uint64_t *keys, int sb);
+
+// ---- brute force (bruteforce.hip)
+void launch_bruteforce(const double *pos, const double *mom, int64_t N, const int64_t *targets,
+                       int64_t nt, double amp, double *E, double *B, hipStream_t st);
+
+struct Ctx {
+    int device = 0;
+    hipStream_t own = nullptr, stream = nullptr;
+    hipEvent_t ev[10] = {};
+    bool debug = false;
+
+    // parameters of the current call
+    int64_t N = 0, N0 = 0;
+    int n = 0, np = 0, nn = 0, precision = 0;
+    Shape shape;
+
+    // shape arrays on the device
+    DevBuf d_lo, d_cnt, d_flags, d_leaves, d_lvl_nodes, d_lvl_split, d_leftbase;
+
+    // tree build scratch
+    DevBuf d_part, d_pavg, d_bad;
+    DevBuf d_keys, d_keys_alt, d_vals, d_vals_alt, d_rs_scratch;
+    DevBuf d_seg, d_seg_alt, d_flag, d_fbuf, d_bsum, d_axis;
+    // tree outputs
+    DevBuf d_parind, d_leafof, d_bbox, d_geo, d_ncoord, d_costab;
+    DevBuf d_sp8, d_sp8f;
+    // lists
+    DevBuf d_front0, d_front1, d_tcnt, d_p2p, d_p2p_alt, d_m2l, d_m2l_alt, d_p2p_off, d_m2l_off;
+    int64_t cap_front = 0, cap_p2p = 0, cap_m2l = 0;
+    int64_t np2p = 0, nm2l = 0;
+    int sb = 1;
+    int trav_iters = 0;
+    uint64_t *p2p_sorted = nullptr, *m2l_sorted = nullptr;
+    // expansions and fields
+    DevBuf d_mp4, d_mp4f, d_loc, d_near;
+    // debug snapshots
+    std::vector<double> snap_loc_m2l;
+    // host-side p_avg of the last call
+    double pavg_host[4] = {0, 0, 0, 1};
+};
+
+}  // namespace fmm4rb

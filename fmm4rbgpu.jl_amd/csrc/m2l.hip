@@ -1,0 +1,169 @@
+// m2l.hip -- multipole-to-local transfer on gfx950 (FP64 and mixed precision).
+//
+// Reference: M2L! (src/fmm/interaction.jl:3-46) and gpu_M2L!
+// (src/gpu_fmm/gpu_interaction.jl:3-41): for every admissible pair (t, s)
+//   E_t(x) += sum_y gamma_s(y) K(x - y; p_avg),  B_t(x) += sum_y m_s(y) x K(x - y; p_avg)
+// over the (n+1)^3 interpolation nodes x of t and y of s, with
+//   K(R; p) = R / sqrt(R.R + (p.R)^2 + eps)^3.
+//
+// One workgroup owns one target cluster and walks its CSR source list; one
+// thread owns one target node.  Each source's multipoles are staged in LDS
+// and read as broadcasts.  The node grid is a tensor product, so for a fixed
+// source row (j, k) the y/z parts of R.R and p.R are hoisted and the i loop
+// needs three operations of geometry per evaluation; E and B are accumulated
+// through the row sums G = sum y3 g, H = sum y3 dx g, A = sum y3 m,
+// U = sum y3 dx m (y3 = d^-3/2), which cost 9 multiply-adds per evaluation
+// instead of 12.  In the mixed-precision form the row geometry is formed in
+// FP64 and rounded once to FP32, the kernel and row sums run in FP32
+// (v_rsq_f32), and each source pair's FP32 sum is added into FP64
+// accumulators.  The reference's LDS reuse without a barrier
+// (gpu_interaction.jl:15-38) is not repeated: a barrier precedes every refill.
+#include "fmm4rb_internal.hpp"
+
+namespace fmm4rb {
+
+template <int NP>
+struct M2LShape {
+    static constexpr int NN = NP * NP * NP;
+    static constexpr int BLK = NN >= 1024 ? 1024 : ((NN + 63) / 64) * 64;
+    static constexpr int TPT = (NN + BLK - 1) / BLK;
+};
+
+template <typename T>
+__device__ inline T rsq(T d);
+template <>
+__device__ inline double rsq<double>(double d) { return rsqrt(d); }
+template <>
+__device__ inline float rsq<float>(float d) { return __builtin_amdgcn_rsqf(d); }
+
+template <int NP, typename T, typename V4>
+__device__ inline void m2l_pair(double tx, double ty, double tz, const double *sn, const V4 *smp,
+                                T px, T py, T pz, double *acc) {
+    T dxr[NP], dx2[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        dxr[i] = (T)(tx - sn[i]);
+        dx2[i] = dxr[i] * dxr[i];
+    }
+    T e0 = 0, e1 = 0, e2 = 0, b0 = 0, b1 = 0, b2 = 0;
+    for (int sk = 0; sk < NP; ++sk) {
+        const T dz = (T)(tz - sn[2 * NP + sk]);
+        for (int sj = 0; sj < NP; ++sj) {
+            const T dy = (T)(ty - sn[NP + sj]);
+            const T rrjk = fma(dy, dy, dz * dz) + (T)EPS64;
+            const T prjk = fma(py, dy, pz * dz);
+            T G = 0, H = 0, Ax = 0, Ay = 0, Az = 0, Uy = 0, Uz = 0;
+            const V4 *row = smp + NP * (sj + NP * sk);
+#pragma unroll
+            for (int si = 0; si < NP; ++si) {
+                const T dx = dxr[si];
+                const T rr = dx2[si] + rrjk;
+                const T pr = fma(px, dx, prjk);
+                const T d = fma(pr, pr, rr);
+                const T y = rsq<T>(d);
+                const T y3 = (y * y) * y;
+                const V4 m = row[si];
+                const T u = y3 * dx;
+                G = fma(y3, m.x, G);
+                H = fma(u, m.x, H);
+                Ax = fma(y3, m.y, Ax);
+                Ay = fma(y3, m.z, Ay);
+                Az = fma(y3, m.w, Az);
+                Uy = fma(u, m.z, Uy);
+                Uz = fma(u, m.w, Uz);
+            }
+            e0 += H;
+            e1 = fma(dy, G, e1);
+            e2 = fma(dz, G, e2);
+            b0 += dz * Ay - dy * Az;
+            b1 += Uz - dz * Ax;
+            b2 += dy * Ax - Uy;
+        }
+    }
+    acc[0] += (double)e0;
+    acc[1] += (double)e1;
+    acc[2] += (double)e2;
+    acc[3] += (double)b0;
+    acc[4] += (double)b1;
+    acc[5] += (double)b2;
+}
+
+template <int NP, bool MIXED>
+__global__ __launch_bounds__(M2LShape<NP>::BLK) void k_m2l(
+    const int32_t *__restrict__ off, const uint64_t *__restrict__ keys, int sb,
+    const double *__restrict__ ncoord, const void *__restrict__ mpv,
+    const double *__restrict__ pavg4, double *__restrict__ loc) {
+    using T = typename std::conditional<MIXED, float, double>::type;
+    using V4 = typename std::conditional<MIXED, float4, double4>::type;
+    constexpr int NN = M2LShape<NP>::NN, BLK = M2LShape<NP>::BLK, TPT = M2LShape<NP>::TPT;
+    __shared__ V4 smp[NN];
+    __shared__ double sn[3 * NP];
+    const V4 *mp = reinterpret_cast<const V4 *>(mpv);
+    const int tid = threadIdx.x;
+    const int32_t t = blockIdx.x;
+    const int32_t beg = off[t], end = off[t + 1];
+    const uint64_t mask = (1ull << sb) - 1ull;
+    const T px = (T)pavg4[0], py = (T)pavg4[1], pz = (T)pavg4[2];
+    double tx[TPT], ty[TPT], tz[TPT], acc[TPT][6];
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        const int node = min(tid + q * BLK, NN - 1);
+        const int i = node % NP, j = (node / NP) % NP, k = node / (NP * NP);
+        tx[q] = ncoord[((int64_t)t * 3 + 0) * NP + i];
+        ty[q] = ncoord[((int64_t)t * 3 + 1) * NP + j];
+        tz[q] = ncoord[((int64_t)t * 3 + 2) * NP + k];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc[q][m] = 0.0;
+    }
+    for (int32_t e = beg; e < end; ++e) {
+        const int64_t s = (int64_t)(keys[e] & mask);
+        __syncthreads();
+        for (int j = tid; j < NN; j += BLK) smp[j] = mp[s * NN + j];
+        if (tid < 3 * NP) sn[tid] = ncoord[s * 3 * NP + tid];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < TPT; ++q)
+            if (tid + q * BLK < NN) m2l_pair<NP, T, V4>(tx[q], ty[q], tz[q], sn, smp, px, py, pz, acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        const int node = tid + q * BLK;
+        if (node < NN) {
+            double *o = loc + ((int64_t)t * NN + node) * 6;
+#pragma unroll
+            for (int m = 0; m < 6; ++m) o[m] = acc[q][m];
+        }
+    }
+}
+
+void launch_m2l(Ctx &c, const int32_t *off, const uint64_t *keys, int sb, int64_t ngroups_grid) {
+    hipStream_t st = c.stream;
+    const bool mixed = c.precision == FMM4RB_MIXED;
+    const void *mp = mixed ? (const void *)c.d_mp4f.p : (const void *)c.d_mp4.p;
+    const unsigned grid = (unsigned)ngroups_grid;
+#define M2L_L(NP)                                                                                \
+    if (mixed)                                                                                   \
+        hipLaunchKernelGGL((k_m2l<NP, true>), dim3(grid), dim3(M2LShape<NP>::BLK), 0, st, off,   \
+                           keys, sb, c.d_ncoord.as<double>(), mp, c.d_pavg.as<double>(),         \
+                           c.d_loc.as<double>());                                                \
+    else                                                                                         \
+        hipLaunchKernelGGL((k_m2l<NP, false>), dim3(grid), dim3(M2LShape<NP>::BLK), 0, st, off,  \
+                           keys, sb, c.d_ncoord.as<double>(), mp, c.d_pavg.as<double>(),         \
+                           c.d_loc.as<double>());
+    switch (c.np) {
+        case 2: M2L_L(2); break;
+        case 3: M2L_L(3); break;
+        case 4: M2L_L(4); break;
+        case 5: M2L_L(5); break;
+        case 6: M2L_L(6); break;
+        case 7: M2L_L(7); break;
+        case 8: M2L_L(8); break;
+        case 9: M2L_L(9); break;
+        case 10: M2L_L(10); break;
+        case 11: M2L_L(11); break;
+        default: fail(FMM4RB_EINVAL, "n out of range");
+    }
+    FMM_LAUNCH_CHECK();
+}
+
+}  // namespace fmm4rb

@@ -1,0 +1,170 @@
+// radix_sort.hip -- stable LSD radix sort of 64-bit keys (with optional
+// 32-bit values) for gfx950, 8-bit digits, three kernels per pass:
+//   hist    : per-tile digit histogram (LDS atomics), digit-major table
+//   scan    : per-digit exclusive scan over tiles + digit totals
+//   scatter : stable in-tile ranking with wave64 ballots, then scatter
+// Several independent arrays of the same length sort in one launch
+// (blockIdx.y selects the array).  Used for the tree's per-axis presort
+// (particle positions) and for grouping interaction pairs by target.
+#include "fmm4rb_internal.hpp"
+
+namespace fmm4rb {
+
+constexpr int RS_THREADS = 256;  // 4 waves of 64
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_BINS = 256;
+
+static inline int64_t rs_nblk(int64_t n) { return (n + RS_TILE - 1) / RS_TILE; }
+
+size_t radix_scratch_words(int64_t n, int nseg) {
+    return (size_t)nseg * RS_BINS * (size_t)rs_nblk(n) + (size_t)nseg * RS_BINS;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restrict__ keys,
+                                                        int64_t n, int shift,
+                                                        uint32_t *__restrict__ hist, int nblk) {
+    __shared__ uint32_t h[RS_BINS];
+    const int seg = blockIdx.y;
+    const uint64_t *k = keys + (int64_t)seg * n;
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        int64_t i = base + r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[(k[i] >> shift) & 0xFF], 1u);
+    }
+    __syncthreads();
+    hist[((int64_t)seg * RS_BINS + threadIdx.x) * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Block-wide inclusive scan of one value per thread (256 threads).
+__device__ inline uint32_t block_incl_scan256(uint32_t v, uint32_t *s) {
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < RS_THREADS; off <<= 1) {
+        uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t r = s[threadIdx.x];
+    __syncthreads();
+    return r;
+}
+
+// Exclusive scan of each row (len entries) of rows[gridDim.x*gridDim.y][len];
+// the row total goes to tot[row].
+__global__ __launch_bounds__(RS_THREADS) void k_scan_rows(uint32_t *__restrict__ rows, int len,
+                                                          uint32_t *__restrict__ tot) {
+    __shared__ uint32_t s[RS_THREADS];
+    __shared__ uint32_t chunk;
+    const int64_t row = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    uint32_t *r = rows + row * len;
+    uint32_t carry = 0;
+    for (int base = 0; base < len; base += RS_THREADS) {
+        int i = base + threadIdx.x;
+        uint32_t v = i < len ? r[i] : 0u;
+        uint32_t incl = block_incl_scan256(v, s);
+        if (i < len) r[i] = carry + incl - v;
+        // the last thread's inclusive value is the chunk total
+        if (threadIdx.x == RS_THREADS - 1) chunk = incl;
+        __syncthreads();
+        carry += chunk;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tot[row] = carry;
+}
+
+template <bool HASVAL>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
+    const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
+    uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ hist, int nblk,
+    const uint32_t *__restrict__ dtot) {
+    __shared__ uint32_t doff[RS_BINS];
+    __shared__ uint32_t run[RS_BINS];
+    __shared__ uint32_t wcnt[RS_THREADS / 64][RS_BINS];
+    __shared__ uint32_t s[RS_THREADS];
+    const int seg = blockIdx.y;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t so = (int64_t)seg * n;
+
+    // digit base = exclusive scan of digit totals + this tile's prefix
+    uint32_t tv = dtot[seg * RS_BINS + tid];
+    uint32_t incl = block_incl_scan256(tv, s);
+    doff[tid] = incl - tv + hist[((int64_t)seg * RS_BINS + tid) * nblk + blockIdx.x];
+    run[tid] = 0;
+    __syncthreads();
+
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const int64_t i = base + r * RS_THREADS + tid;
+        const bool valid = i < n;
+        const uint64_t key = valid ? kin[so + i] : 0ull;
+        uint32_t val = 0;
+        if (HASVAL && valid) val = vin[so + i];
+        const uint32_t dg = (uint32_t)(key >> shift) & 0xFFu;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (dg >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        for (int j = tid; j < (RS_THREADS / 64) * RS_BINS; j += RS_THREADS) (&wcnt[0][0])[j] = 0;
+        __syncthreads();
+        if (valid && (peers >> lane) == 1ull) wcnt[wave][dg] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = run[dg] + (uint32_t)__popcll(peers & lt);
+            for (int w = 0; w < wave; ++w) off += wcnt[w][dg];
+            const uint32_t pos = doff[dg] + off;
+            kout[so + pos] = key;
+            if (HASVAL) vout[so + pos] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < RS_THREADS / 64; ++w) add += wcnt[w][tid];
+        run[tid] += add;
+        __syncthreads();
+    }
+}
+
+void scan_rows(uint32_t *rows, int nrows, int len, uint32_t *tot, hipStream_t st) {
+    hipLaunchKernelGGL(k_scan_rows, dim3(nrows), dim3(RS_THREADS), 0, st, rows, len, tot);
+    FMM_LAUNCH_CHECK();
+}
+
+bool radix_sort(uint64_t *keys, uint64_t *keys_alt, uint32_t *vals, uint32_t *vals_alt, int64_t n,
+                int nseg, int bit0, int bit1, uint32_t *scratch, hipStream_t st) {
+    if (n <= 0 || bit1 <= bit0) return false;
+    const int nblk = (int)rs_nblk(n);
+    uint32_t *hist = scratch;
+    uint32_t *dtot = scratch + (size_t)nseg * RS_BINS * nblk;
+    bool alt = false;
+    for (int shift = bit0; shift < bit1; shift += 8) {
+        uint64_t *ki = alt ? keys_alt : keys, *ko = alt ? keys : keys_alt;
+        uint32_t *vi = alt ? vals_alt : vals, *vo = alt ? vals : vals_alt;
+        dim3 grid(nblk, nseg);
+        hipLaunchKernelGGL(k_rs_hist, grid, dim3(RS_THREADS), 0, st, ki, n, shift, hist, nblk);
+        FMM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_scan_rows, dim3(RS_BINS, nseg), dim3(RS_THREADS), 0, st, hist, nblk,
+                           dtot);
+        FMM_LAUNCH_CHECK();
+        if (vals)
+            hipLaunchKernelGGL(k_rs_scatter<true>, grid, dim3(RS_THREADS), 0, st, ki, vi, ko, vo, n,
+                               shift, hist, nblk, dtot);
+        else
+            hipLaunchKernelGGL(k_rs_scatter<false>, grid, dim3(RS_THREADS), 0, st, ki,
+                               (const uint32_t *)nullptr, ko, (uint32_t *)nullptr, n, shift, hist,
+                               nblk, dtot);
+        FMM_LAUNCH_CHECK();
+        alt = !alt;
+    }
+    return alt;
+}
+
+}  // namespace fmm4rb

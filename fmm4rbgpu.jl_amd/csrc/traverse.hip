@@ -1,0 +1,193 @@
+// traverse.hip -- dual-tree traversal into flat P2P / M2L lists on gfx950.
+//
+// Reference: dualtraversefill! and admissible (src/fmm/interaction_lists.jl:10-49),
+// diameter (src/utils/bbox.jl:9-11), and the target-grouped CSR of
+// InteractionListsGPU (src/gpu_fmm/gpu_interaction_lists.jl:14-40).
+//
+// The reference recursion from (root, root) decides each pair (t, s) from
+// t's and s's data alone, so the visited pairs form a tree whose levels can
+// be expanded breadth-first: iteration i holds the pairs whose depths sum to
+// i, and at most 2*maxlevel+1 iterations run.  Each pair becomes a P2P pair
+// (both leaves), an M2L pair (admissible), or two child pairs (split the leaf
+// -less side, else the side with the larger diameter, ties split the source).
+// Outputs are appended with wave-aggregated atomics as keys (t << sb) | s,
+// then radix-sorted, which gives the canonical order (target, then source)
+// and the CSR offsets per target node.
+#include "fmm4rb_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace fmm4rb {
+
+constexpr int TR_THREADS = 256;
+constexpr int TR_BLOCKS = 2048;
+
+__device__ inline double norm3(double a, double b, double c) { return sqrt((a * a + b * b) + c * c); }
+
+// interaction_lists.jl:10-21 with precomputed centre and radius per node.
+__device__ inline bool admissible(const double *gt, const double *gs, double g, double eta) {
+    double v0 = 1.0 * (gt[0] - gs[0]);
+    double v1 = 1.0 * (gt[1] - gs[1]);
+    double v2 = g * (gt[2] - gs[2]);
+    double R = norm3(v0, v1, v2);
+    double r = gt[3] < gs[3] ? gs[3] : gt[3];
+    return (r / R) < eta;
+}
+
+__global__ void k_trav_init(int2 *front, uint32_t *cnt, int ncnt) {
+    int i = threadIdx.x;
+    for (int j = i; j < ncnt; j += blockDim.x) cnt[j] = 0;
+    if (i == 0) {
+        front[0] = make_int2(0, 0);
+    }
+    __syncthreads();
+    if (i == 0) cnt[2] = 1;
+}
+
+__global__ __launch_bounds__(TR_THREADS) void k_traverse(
+    const int2 *__restrict__ fin, int2 *__restrict__ fout, uint32_t *__restrict__ cnt, int it,
+    uint32_t cap_f, uint64_t *__restrict__ p2p, uint32_t cap_p2p, uint64_t *__restrict__ m2l,
+    uint32_t cap_m2l, int sb, const double *__restrict__ geo, const uint8_t *__restrict__ flags,
+    const double *__restrict__ pavg4, double eta) {
+    const uint32_t n = min(cnt[2 + it], cap_f);
+    const double g = pavg4[3];
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (uint32_t base = blockIdx.x * TR_THREADS; base < n; base += gridDim.x * TR_THREADS) {
+        const uint32_t i = base + threadIdx.x;
+        int cat = -1, t = 0, s = 0;
+        if (i < n) {
+            int2 pr = fin[i];
+            t = pr.x;
+            s = pr.y;
+            const bool tl = flags[t] & 2, sl = flags[s] & 2;
+            const double *gt = geo + 8 * (int64_t)t, *gs = geo + 8 * (int64_t)s;
+            if (tl && sl) cat = 0;
+            else if (admissible(gt, gs, g, eta)) cat = 1;
+            else if (tl) cat = 3;
+            else if (sl) cat = 2;
+            else cat = gt[4] > gs[4] ? 2 : 3;  // diameter(t) > diameter(s)
+        }
+        const uint64_t m0 = __ballot(cat == 0), m1 = __ballot(cat == 1), m2 = __ballot(cat >= 2);
+        uint32_t b0 = 0, b1 = 0, b2 = 0;
+        if (lane == 0) {
+            if (m0) b0 = atomicAdd(&cnt[0], (uint32_t)__popcll(m0));
+            if (m1) b1 = atomicAdd(&cnt[1], (uint32_t)__popcll(m1));
+            if (m2) b2 = atomicAdd(&cnt[2 + it + 1], 2u * (uint32_t)__popcll(m2));
+        }
+        b0 = __shfl(b0, 0);
+        b1 = __shfl(b1, 0);
+        b2 = __shfl(b2, 0);
+        if (cat == 0) {
+            uint32_t o = b0 + (uint32_t)__popcll(m0 & lt);
+            if (o < cap_p2p) p2p[o] = ((uint64_t)t << sb) | (uint64_t)s;
+        } else if (cat == 1) {
+            uint32_t o = b1 + (uint32_t)__popcll(m1 & lt);
+            if (o < cap_m2l) m2l[o] = ((uint64_t)t << sb) | (uint64_t)s;
+        } else if (cat >= 2) {
+            uint32_t o = b2 + 2u * (uint32_t)__popcll(m2 & lt);
+            if (o + 1 < cap_f) {
+                if (cat == 2) {
+                    fout[o] = make_int2(2 * t + 1, s);
+                    fout[o + 1] = make_int2(2 * t + 2, s);
+                } else {
+                    fout[o] = make_int2(t, 2 * s + 1);
+                    fout[o + 1] = make_int2(t, 2 * s + 2);
+                }
+            }
+        }
+    }
+}
+
+// off[t] = first index whose target is >= t (binary search), t in [0, nc].
+__global__ void k_csr(const uint64_t *__restrict__ keys, int64_t n, int sb, int64_t nc,
+                      int32_t *__restrict__ off) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > nc) return;
+    const uint64_t key = (uint64_t)t << sb;
+    int64_t a = 0, b = n;
+    while (a < b) {
+        int64_t m = (a + b) >> 1;
+        if (keys[m] < key) a = m + 1;
+        else b = m;
+    }
+    off[t] = (int32_t)a;
+}
+
+void sort_pairs_csr(Ctx &c, uint64_t *keys, uint64_t *keys_alt, int64_t n, int32_t *off,
+                    uint64_t **sorted) {
+    hipStream_t st = c.stream;
+    bool alt = false;
+    if (n > 0)
+        alt = radix_sort(keys, keys_alt, nullptr, nullptr, n, 1, 0, 2 * c.sb,
+                         c.d_rs_scratch.as<uint32_t>(), st);
+    uint64_t *r = alt ? keys_alt : keys;
+    *sorted = r;
+    const int64_t nc = c.shape.nc;
+    hipLaunchKernelGGL(k_csr, dim3((unsigned)((nc + 1 + 255) / 256)), dim3(256), 0, st, r, n, c.sb,
+                       nc, off);
+    FMM_LAUNCH_CHECK();
+}
+
+void build_lists_device(Ctx &c, double eta) {
+    const Shape &S = c.shape;
+    hipStream_t st = c.stream;
+    const int iters = 2 * S.ml + 1;
+    const int ncnt = iters + 3;
+    for (int attempt = 0;; ++attempt) {
+        int2 *f0 = c.d_front0.as<int2>(), *f1 = c.d_front1.as<int2>();
+        uint32_t *cnt = c.d_tcnt.as<uint32_t>();
+        hipLaunchKernelGGL(k_trav_init, dim3(1), dim3(64), 0, st, f0, cnt, ncnt);
+        FMM_LAUNCH_CHECK();
+        for (int it = 0; it < iters; ++it) {
+            const int2 *fin = (it & 1) ? f1 : f0;
+            int2 *fout = (it & 1) ? f0 : f1;
+            hipLaunchKernelGGL(k_traverse, dim3(TR_BLOCKS), dim3(TR_THREADS), 0, st, fin, fout, cnt,
+                               it, (uint32_t)c.cap_front, c.d_p2p.as<uint64_t>(),
+                               (uint32_t)c.cap_p2p, c.d_m2l.as<uint64_t>(), (uint32_t)c.cap_m2l,
+                               c.sb, c.d_geo.as<double>(), c.d_flags.as<uint8_t>(),
+                               c.d_pavg.as<double>(), eta);
+            FMM_LAUNCH_CHECK();
+        }
+        std::vector<uint32_t> h(ncnt);
+        FMM_HIP(hipMemcpyAsync(h.data(), cnt, sizeof(uint32_t) * ncnt, hipMemcpyDeviceToHost, st));
+        FMM_HIP(hipStreamSynchronize(st));
+        uint64_t maxf = 0;
+        for (int it = 0; it <= iters; ++it) maxf = std::max<uint64_t>(maxf, h[2 + it]);
+        bool over = h[0] > (uint64_t)c.cap_p2p || h[1] > (uint64_t)c.cap_m2l ||
+                    maxf > (uint64_t)c.cap_front;
+        if (!over) {
+            c.np2p = h[0];
+            c.nm2l = h[1];
+            c.trav_iters = iters;
+            break;
+        }
+        if (attempt >= 8) fail(FMM4RB_ENOMEM, "interaction list capacity did not converge");
+        // grow what overflowed (2x the count seen) and traverse again
+        if (h[0] >= (uint64_t)c.cap_p2p) {
+            c.cap_p2p = std::max<int64_t>(2 * (int64_t)h[0], 2 * c.cap_p2p);
+            c.d_p2p.ensure(8 * (size_t)c.cap_p2p);
+            c.d_p2p_alt.ensure(8 * (size_t)c.cap_p2p);
+        }
+        if (h[1] >= (uint64_t)c.cap_m2l) {
+            c.cap_m2l = std::max<int64_t>(2 * (int64_t)h[1], 2 * c.cap_m2l);
+            c.d_m2l.ensure(8 * (size_t)c.cap_m2l);
+            c.d_m2l_alt.ensure(8 * (size_t)c.cap_m2l);
+        }
+        if (maxf >= (uint64_t)c.cap_front) {
+            c.cap_front = std::max<int64_t>(2 * (int64_t)maxf, 2 * c.cap_front);
+            c.d_front0.ensure(8 * (size_t)c.cap_front);
+            c.d_front1.ensure(8 * (size_t)c.cap_front);
+        }
+        if (c.cap_front > 0x7fffffffLL || c.cap_p2p > 0x7fffffffLL || c.cap_m2l > 0x7fffffffLL)
+            fail(FMM4RB_ENOMEM, "interaction lists exceed 2^31 entries");
+    }
+    // the sort scratch must also cover the longer list (the stream is idle here)
+    c.d_rs_scratch.ensure(4 * radix_scratch_words(std::max<int64_t>(std::max(c.np2p, c.nm2l), 1), 1));
+    sort_pairs_csr(c, c.d_p2p.as<uint64_t>(), c.d_p2p_alt.as<uint64_t>(), c.np2p,
+                   c.d_p2p_off.as<int32_t>(), &c.p2p_sorted);
+    sort_pairs_csr(c, c.d_m2l.as<uint64_t>(), c.d_m2l_alt.as<uint64_t>(), c.nm2l,
+                   c.d_m2l_off.as<int32_t>(), &c.m2l_sorted);
+}
+
+}  // namespace fmm4rb

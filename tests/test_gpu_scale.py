@@ -1,0 +1,65 @@
+"""Size-independent properties at the full configurations (no oracle run):
+the tree's ranges and boxes, the every-pair-exactly-once checksum of the
+interaction lists, FMM accuracy against the GPU BruteForce on sampled
+targets, mixed vs FP64, and determinism."""
+import numpy as np
+import pytest
+
+from fmm4rb_hip import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def bunch20():
+    return W.bunch(1 << 20, seed=2)
+
+
+def test_tree_invariants_2e20(ctx, bunch20):
+    pos, mom = bunch20
+    N, N0 = len(pos), 125
+    t = ctx.tree(pos, mom, N0)
+    pi, lo, hi, ch, bb = t["parindices"], t["lo"], t["hi"], t["children"], t["bboxes"]
+    assert np.array_equal(np.sort(pi), np.arange(N))
+    leaves = np.nonzero((lo >= 0) & (ch[:, 0] < 0))[0]
+    assert len(leaves) == 1 << t["ml"]
+    assert np.all(hi[leaves] - lo[leaves] + 1 <= N0)
+    x = pos[pi]
+    mn = np.minimum.reduceat(x, lo[leaves], axis=0)
+    mx = np.maximum.reduceat(x, lo[leaves], axis=0)
+    assert np.array_equal(mn + 0.0, bb[leaves, :3]) and np.array_equal(mx + 0.0, bb[leaves, 3:])
+    inner = np.nonzero(ch[:, 0] >= 0)[0]
+    for c in (ch[inner, 0], ch[inner, 1]):
+        assert np.all(bb[c, :3] >= bb[inner, :3]) and np.all(bb[c, 3:] <= bb[inner, 3:])
+        assert np.all(lo[c] >= lo[inner]) and np.all(hi[c] <= hi[inner])
+
+
+def test_lists_cover_every_pair_once_2e20(ctx, bunch20):
+    pos, mom = bunch20
+    N, N0 = len(pos), 125
+    t = ctx.tree(pos, mom, N0)
+    cnt = (t["hi"] - t["lo"] + 1).astype(np.float64)
+    p2p, m2l = ctx.lists(pos, mom, N0, 0.5)
+    total = (cnt[p2p[:, 0]] * cnt[p2p[:, 1]]).sum() + (cnt[m2l[:, 0]] * cnt[m2l[:, 1]]).sum()
+    assert total == float(N) * float(N)
+    # every leaf interacts with itself, lists are sorted and duplicate-free
+    leaves = np.nonzero((t["lo"] >= 0) & (t["children"][:, 0] < 0))[0]
+    selfp = p2p[p2p[:, 0] == p2p[:, 1], 0]
+    assert np.array_equal(np.sort(selfp), leaves)
+    for a in (p2p, m2l):
+        k = a[:, 0] * (1 << 32) + a[:, 1]
+        assert np.all(np.diff(k) > 0)
+
+
+@pytest.mark.parametrize("n", [4, 6])
+def test_accuracy_and_precision_2e20(ctx, bunch20, n):
+    pos, mom = bunch20
+    tg = np.random.default_rng(40).choice(len(pos), 4096, replace=False)
+    Eb, Bb = ctx.bruteforce(pos, mom, -1.0, 1e-3, targets=tg)
+    E, B, st = ctx.field(pos, mom, -1.0, 1e-3, n, 125, 0.5, "fp64")
+    Em, Bm, _ = ctx.field(pos, mom, -1.0, 1e-3, n, 125, 0.5, "mixed")
+    assert W.rel_l2(E[tg], Eb) < 1e-2 and W.rel_l2(B[tg], Bb) < 1e-2
+    assert W.rel_l2(Em, E) <= 1e-5 and W.rel_l2(Bm, B) <= 1e-5
+    print(f"n={n} fp64-vs-BF E {W.rel_l2(E[tg], Eb):.3e} B {W.rel_l2(B[tg], Bb):.3e}; "
+          f"mixed-vs-fp64 E {W.rel_l2(Em, E):.3e} B {W.rel_l2(Bm, B):.3e}; "
+          f"np2p {st.np2p} nm2l {st.nm2l} ms {st.ms_total:.1f}")
