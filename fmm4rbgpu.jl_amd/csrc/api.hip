@@ -165,7 +165,7 @@ static void prepare(Ctx &c, int64_t N, int64_t N0, int n, int precision) {
     c.d_geo.ensure(8 * nc * 8);
     c.d_ncoord.ensure(3 * (size_t)c.np * nc * 8);
     c.d_sp8.ensure(8 * (size_t)N * 8);
-    if (precision == FMM4RB_MIXED) c.d_sp8f.ensure(8 * (size_t)N * 4);
+    if (precision == FMM4RB_MIXED) c.d_sp8f.ensure(4 * (size_t)N * 4);
     if (c.cap_front == 0) {
         c.cap_front = std::max<int64_t>(1 << 16, 48 * nleaf);
         c.cap_p2p = std::max<int64_t>(1 << 16, 64 * nleaf);
@@ -642,39 +642,33 @@ int fmm4rb_op_p2p(fmm4rb_ctx *ctx, int precision, int64_t N, const double *pos, 
     c.N = N;
     Shape &S = c.shape;
     FMM_NEED(S.maxleaf <= MAX_N0, "cluster larger than 1024 particles");
-    // targets of the listed pairs act as the leaves; positions map to clusters
+    // the targets of the listed pairs act as the leaves
     std::vector<uint8_t> seen((size_t)nc, 0);
-    std::vector<int32_t> leafof((size_t)N, 0);
     S.leaves.clear();
     for (int64_t i = 0; i < npairs; ++i)
         for (int side = 0; side < 2; ++side) {
             const int64_t k = pairs[2 * i + side];
             FMM_NEED(k >= 0 && k < nc && S.lo[k] >= 0, "pair refers to a cluster without range");
-            for (int64_t p = S.lo[k]; p < (int64_t)S.lo[k] + S.cnt[k]; ++p) leafof[p] = (int32_t)k;
             if (side == 0 && !seen[k]) {
                 seen[k] = 1;
                 S.leaves.push_back((int32_t)k);
             }
         }
+    (void)bboxes;  // cluster boxes are not needed: R is formed from absolute positions
     std::vector<double> sp(8 * (size_t)N);
-    std::vector<float> spf(8 * (size_t)N);
-    std::vector<double> bb(6 * (size_t)nc, 0.0);
-    if (bboxes) std::memcpy(bb.data(), bboxes, 48 * (size_t)nc);
+    std::vector<float> spf(4 * (size_t)N);
     for (int64_t i = 0; i < N; ++i) {
         const int64_t q = parindices[i];
         const double *m = mom + 3 * q;
         const double g = std::sqrt(1.0 + ((m[0] * m[0] + m[1] * m[1]) + m[2] * m[2]));
-        const double *b = bb.data() + 6 * leafof[i];
         for (int d = 0; d < 3; ++d) {
             sp[8 * i + d] = pos[3 * q + d];
             sp[8 * i + 4 + d] = m[d];
-            spf[8 * i + d] = (float)(pos[3 * q + d] - (b[d] + b[3 + d]) / 2.0);
-            spf[8 * i + 4 + d] = (float)m[d];
+            spf[4 * i + 1 + d] = (float)m[d];
         }
         sp[8 * i + 3] = g;
-        spf[8 * i + 3] = (float)g;
+        spf[4 * i] = (float)g;
         sp[8 * i + 7] = 0.0;
-        spf[8 * i + 7] = 0.0f;
     }
     upload(c.d_lo, S.lo, st);
     upload(c.d_cnt, S.cnt, st);
@@ -683,20 +677,9 @@ int fmm4rb_op_p2p(fmm4rb_ctx *ctx, int precision, int64_t N, const double *pos, 
     c.d_sp8.ensure(8 * sp.size());
     c.d_sp8f.ensure(4 * spf.size());
     c.d_near.ensure(6 * (size_t)N * 8);
-    c.d_bbox.ensure(48 * (size_t)nc);
-    c.d_geo.ensure(64 * (size_t)nc);
-    c.d_ncoord.ensure(3 * 2 * (size_t)nc * 8);
-    c.d_pavg.ensure(32);
-    const double pa[4] = {0.0, 0.0, 0.0, 1.0};
-    c.n = 1;
-    c.np = 2;
-    c.nn = 8;
-    FMM_HIP(hipMemcpyAsync(c.d_pavg.p, pa, 32, hipMemcpyHostToDevice, st));
-    FMM_HIP(hipMemcpyAsync(c.d_bbox.p, bb.data(), 48 * (size_t)nc, hipMemcpyHostToDevice, st));
     FMM_HIP(hipMemcpyAsync(c.d_sp8.p, sp.data(), 8 * sp.size(), hipMemcpyHostToDevice, st));
     FMM_HIP(hipMemcpyAsync(c.d_sp8f.p, spf.data(), 4 * spf.size(), hipMemcpyHostToDevice, st));
     FMM_HIP(hipMemsetAsync(c.d_near.p, 0, 6 * (size_t)N * 8, st));
-    launch_node_geo(c);  // cluster centres for the mixed-precision offsets
     uint64_t *sorted = nullptr;
     hook_pairs(c, nc, npairs, pairs, c.d_p2p, c.d_p2p_alt, c.d_p2p_off, &sorted);
     c.cap_p2p = std::max<int64_t>(c.cap_p2p, (int64_t)(c.d_p2p.cap / 8));

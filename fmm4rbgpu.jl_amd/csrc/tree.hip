@@ -261,11 +261,10 @@ __global__ __launch_bounds__(256) void k_leaf_sort(const int32_t *__restrict__ l
 }
 
 // Particles in tree order: sp8 = (x, y, z, gamma, px, py, pz, 0) and, for the
-// mixed-precision path, sp8f with positions relative to the leaf centre.
+// mixed-precision path, sp4f = (gamma, px, py, pz) in FP32.
 __global__ void k_gather(int64_t N, const int32_t *__restrict__ parind,
-                         const int32_t *__restrict__ leafof, const double *__restrict__ pos,
-                         const double *__restrict__ mom, const double *__restrict__ bbox,
-                         double *__restrict__ sp8, float *__restrict__ sp8f) {
+                         const double *__restrict__ pos, const double *__restrict__ mom,
+                         double *__restrict__ sp8, float *__restrict__ sp4f) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     int64_t p = parind[i];
@@ -275,15 +274,8 @@ __global__ void k_gather(int64_t N, const int32_t *__restrict__ parind,
     double4 *o = reinterpret_cast<double4 *>(sp8 + 8 * i);
     o[0] = make_double4(x0, x1, x2, g);
     o[1] = make_double4(m0, m1, m2, 0.0);
-    if (sp8f) {
-        const double *bb = bbox + 6 * (int64_t)leafof[i];
-        float u0 = (float)(x0 - (bb[0] + bb[3]) / 2.0);
-        float u1 = (float)(x1 - (bb[1] + bb[4]) / 2.0);
-        float u2 = (float)(x2 - (bb[2] + bb[5]) / 2.0);
-        float4 *of = reinterpret_cast<float4 *>(sp8f + 8 * i);
-        of[0] = make_float4(u0, u1, u2, (float)g);
-        of[1] = make_float4((float)m0, (float)m1, (float)m2, 0.0f);
-    }
+    if (sp4f)
+        reinterpret_cast<float4 *>(sp4f)[i] = make_float4((float)g, (float)m0, (float)m1, (float)m2);
 }
 
 // ------------------------------------------------------------- host side
@@ -338,8 +330,7 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
                        cnt, L, c.d_parind.as<int32_t>(), c.d_leafof.as<int32_t>());
     FMM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_gather, dim3(nb(N, 256)), dim3(256), 0, st, N, c.d_parind.as<int32_t>(),
-                       c.d_leafof.as<int32_t>(), pos, mom, c.d_bbox.as<double>(),
-                       c.d_sp8.as<double>(),
+                       pos, mom, c.d_sp8.as<double>(),
                        c.precision == FMM4RB_MIXED ? c.d_sp8f.as<float>() : (float *)nullptr);
     FMM_LAUNCH_CHECK();
 }

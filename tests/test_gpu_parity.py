@@ -48,7 +48,9 @@ def _oracle_tree(orc, pos, N0, pavg):
 def test_tree_bit_exact(ctx, orc, name):
     (pos, mom), N0 = _case(name)
     g = ctx.tree(pos, mom, N0)
-    np.testing.assert_allclose(g["p_avg"], orc.p_avg(mom), rtol=1e-13, atol=1e-300)
+    # summation order differs: compare at the scale of the summands
+    scale = np.abs(mom).mean(axis=0) + 1e-300
+    assert np.all(np.abs(g["p_avg"] - orc.p_avg(mom)) <= 1e-13 * scale)
     o = _oracle_tree(orc, pos, N0, g["p_avg"])
     assert np.array_equal(g["parindices"], o["parindices"])
     assert np.array_equal(g["lo"], o["lo"]) and np.array_equal(g["hi"], o["hi"])
