@@ -20,6 +20,10 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 METRIC = "particles/s field eval N=2^22: mixed-prec vs FP64 HIP FMM at rel err <=1e-5"
 PEAK = {"mixed": 157.3e12, "fp64": 78.6e12}  # MI355X FP32 / FP64 vector peaks (FLOP/s)
 FLOPS_PER_PAIR = 40  # one kernel evaluation, P2P particle pair or M2L node pair
+# HBM bytes per launch from two rocprofv3 PMC passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000,
+# calibrated on a 1 GiB copy), N=2^22, n=4: profiles/m2l_p2p_bunch_2e22_n4_traffic.md
+TRAFFIC = {(1 << 22, 4, "mixed", "m2l"): 4.80e9, (1 << 22, 4, "mixed", "p2p"): 12.44e9,
+           (1 << 22, 4, "fp64", "m2l"): 9.44e9, (1 << 22, 4, "fp64", "p2p"): 8.31e9}
 
 
 def parse():
@@ -34,6 +38,9 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 17,
                     help="particles of the serial CPU FMM sample (0 = skip)")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a known-size "
+                         "kernel that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -59,7 +66,7 @@ def stage_avg(stats):
     return {k[3:]: sum(getattr(s, k) for s in stats) / len(stats) for k in keys}
 
 
-def roofline(st, stages, n, prec):
+def roofline(st, stages, n, prec, N):
     nn = (n + 1) ** 3
     m2l_flops = FLOPS_PER_PAIR * st.nm2l * nn * nn
     p2p_flops = FLOPS_PER_PAIR * st.p2p_particle_pairs
@@ -70,7 +77,8 @@ def roofline(st, stages, n, prec):
     other = "p2p" if name == "m2l" else "m2l"
     of, oms = kern[other]
     return {"kernel": name, "bound": "valu", "achieved": ach / 1e12, "peak": PEAK[prec] / 1e12,
-            "unit": "TFLOP/s", "frac": ach / PEAK[prec], "traffic": None,
+            "unit": "TFLOP/s", "frac": ach / PEAK[prec],
+            "traffic": TRAFFIC.get((N, n, prec, name)),
             "flops_per_launch": flops, "avg_launch_ms": ms,
             "other": {"kernel": other, "achieved": of / (oms * 1e-3) / 1e12,
                       "frac": of / (oms * 1e-3) / PEAK[prec], "avg_launch_ms": oms}}
@@ -116,7 +124,7 @@ def main():
                            lam)
         res["fp64"] = {"value": a.N / ms_d, "ms_per_step": ms_d * 1e3,
                        "stages_ms": stage_avg(st_d),
-                       "roofline": roofline(st_d[-1], stage_avg(st_d), a.n, "fp64")}
+                       "roofline": roofline(st_d[-1], stage_avg(st_d), a.n, "fp64", a.N)}
     # accuracy, outside the timed region
     Em, Bm = bufs_m[2].cpu().numpy(), bufs_m[3].cpu().numpy()
     acc = {}
@@ -144,12 +152,18 @@ def main():
         "counts": {"ncluster": s.ncluster, "nleaf": s.nleaf, "np2p": s.np2p, "nm2l": s.nm2l,
                    "p2p_particle_pairs": s.p2p_particle_pairs, "maxlevel": s.maxlevel},
         "stages_ms": stages,
-        "roofline": roofline(s, stages, a.n, "mixed"),
+        "roofline": roofline(s, stages, a.n, "mixed", a.N),
     }
     if "fp64" in res:
         out["fp64"] = res["fp64"]
         out["speedup_vs_fp64"] = out["value"] / res["fp64"]["value"]
     ctx.close()
+    if a.copy_probe_mib > 0:
+        src = torch.ones(a.copy_probe_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = src.numel() * 4
     if a.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(a.cpu_sample, a.n, a.N0, a.eta)
     print(json.dumps(out))

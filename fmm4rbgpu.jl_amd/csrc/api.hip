@@ -186,19 +186,34 @@ static void prepare(Ctx &c, int64_t N, int64_t N0, int n, int precision) {
     c.d_near.ensure(6 * (size_t)N * 8);
 }
 
-__global__ void k_list_stats(const uint64_t *__restrict__ p2p, int64_t np2p,
-                             const uint64_t *__restrict__ m2l, int64_t nm2l, int sb,
-                             const int32_t *__restrict__ cnt, unsigned long long *__restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_list_stats(const uint64_t *__restrict__ p2p, int64_t np2p,
+                                                    const uint64_t *__restrict__ m2l, int64_t nm2l,
+                                                    int sb, const int32_t *__restrict__ cnt,
+                                                    unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long s[2][256];
     const uint64_t mask = (1ull << sb) - 1ull;
     unsigned long long pp = 0, g = 0;
-    if (i < np2p) {
-        const uint64_t k = p2p[i];
-        pp = (unsigned long long)cnt[k >> sb] * (unsigned long long)cnt[k & mask];
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < np2p || i < nm2l;
+         i += (int64_t)gridDim.x * 256) {
+        if (i < np2p) {
+            const uint64_t k = p2p[i];
+            pp += (unsigned long long)cnt[k >> sb] * (unsigned long long)cnt[k & mask];
+        }
+        if (i < nm2l && (i == 0 || (m2l[i] >> sb) != (m2l[i - 1] >> sb))) g += 1;
     }
-    if (i < nm2l && (i == 0 || (m2l[i] >> sb) != (m2l[i - 1] >> sb))) g = 1;
-    if (pp) atomicAdd(&out[0], pp);
-    if (g) atomicAdd(&out[1], g);
+    s[0][threadIdx.x] = pp;
+    s[1][threadIdx.x] = g;
+    for (int off = 128; off > 0; off >>= 1) {
+        __syncthreads();
+        if (threadIdx.x < (unsigned)off) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + off];
+            s[1][threadIdx.x] += s[1][threadIdx.x + off];
+        }
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&out[0], s[0][0]);
+        atomicAdd(&out[1], s[1][0]);
+    }
 }
 
 // p_avg and the input check; returns after the check (one host sync).
@@ -253,7 +268,8 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
         FMM_HIP(hipMemsetAsync(tmp.p, 0, 16, st));
         const int64_t mx = std::max(c.np2p, c.nm2l);
         if (mx > 0) {
-            hipLaunchKernelGGL(k_list_stats, dim3((unsigned)((mx + 255) / 256)), dim3(256), 0, st,
+            hipLaunchKernelGGL(k_list_stats, dim3((unsigned)std::min<int64_t>(1024, (mx + 255) / 256)),
+                               dim3(256), 0, st,
                                c.p2p_sorted, c.np2p, c.m2l_sorted, c.nm2l, c.sb,
                                c.d_cnt.as<int32_t>(), tmp.as<unsigned long long>());
             FMM_LAUNCH_CHECK();

@@ -1,0 +1,90 @@
+# FMM4RBHIP.jl -- Julia binding of libfmm4rb_hip.so (gfx950 FMM, C ABI in include/fmm4rb.h).
+#
+# Adds the algorithm type `FMMHIP` with the reference's (n, N0, eta) fields
+# (src/update_particles_field.jl:9-19) plus a precision, and a method of
+# `update_particles_field!` for it.  When FMM4RBGPU is loaded the method extends
+# FMM4RBGPU.update_particles_field!; otherwise the module defines a standalone
+# `Particles`-compatible entry point `update_fields!`.
+#
+# Julia is not installed where this library is built and tested; the C ABI is
+# what the test suite exercises (tests/test_gpu_parity.py through ctypes).
+module FMM4RBHIP
+
+export FMMHIP, update_fields!
+
+const libfmm4rb = get(ENV, "FMM4RB_HIP_LIB",
+                      joinpath(@__DIR__, "..", "lib", "libfmm4rb_hip.so"))
+
+const FP64 = Cint(0)
+const MIXED = Cint(1)
+
+Base.@kwdef struct FMMHIP{T}
+    n::Int
+    N0::Int
+    eta::T
+    precision::Symbol = :fp64   # :fp64 or :mixed
+end
+
+mutable struct Context
+    ptr::Ptr{Cvoid}
+end
+
+function last_error()
+    unsafe_string(ccall((:fmm4rb_last_error, libfmm4rb), Cstring, ()))
+end
+
+check(rc) = rc == 0 || error("fmm4rb error $rc: $(last_error())")
+
+function Context(device::Integer = 0)
+    ref = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:fmm4rb_create, libfmm4rb), Cint, (Ptr{Ptr{Cvoid}}, Cint), ref, device))
+    ctx = Context(ref[])
+    finalizer(c -> ccall((:fmm4rb_destroy, libfmm4rb), Cint, (Ptr{Cvoid},), c.ptr), ctx)
+    return ctx
+end
+
+const DEFAULT = Ref{Union{Nothing,Context}}(nothing)
+default_context() = something(DEFAULT[], (DEFAULT[] = Context(0)))
+
+precision_code(p::Symbol) = p === :fp64 ? FP64 : p === :mixed ? MIXED :
+                            throw(ArgumentError("precision must be :fp64 or :mixed"))
+
+"""
+    update_fields!(positions, momenta, efields, bfields, charge, alg::FMMHIP; lambda)
+
+Vectors of SVector{3,Float64} (isbits, contiguous): the fields are overwritten,
+as FMMGPU does (src/update_particles_field.jl:74-75,146-147).
+"""
+function update_fields!(positions::AbstractVector, momenta::AbstractVector,
+                        efields::AbstractVector, bfields::AbstractVector,
+                        charge::Real, alg::FMMHIP; lambda, ctx::Context = default_context())
+    N = length(positions)
+    @assert length(momenta) == N && length(efields) == N && length(bfields) == N
+    pos = reinterpret(Float64, positions)
+    mom = reinterpret(Float64, momenta)
+    E = reinterpret(Float64, efields)
+    B = reinterpret(Float64, bfields)
+    GC.@preserve pos mom E B begin
+        check(ccall((:fmm4rb_field, libfmm4rb), Cint,
+                    (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Cint, Int64,
+                     Float64, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
+                    ctx.ptr, N, pointer(pos), pointer(mom), Float64(charge), Float64(lambda),
+                    Cint(alg.n), Int64(alg.N0), Float64(alg.eta), precision_code(alg.precision),
+                    pointer(E), pointer(B), C_NULL))
+    end
+    return nothing
+end
+
+# Extend the reference's generic function when the reference package is present.
+function __init__()
+    if isdefined(Main, :FMM4RBGPU)
+        FM = getfield(Main, :FMM4RBGPU)
+        @eval function $(FM).update_particles_field!(particles::$(FM).Particles{Float64},
+                                                     alg::FMMHIP; lambda)
+            update_fields!(particles.positions, particles.momenta, particles.efields,
+                           particles.bfields, particles.charge, alg; lambda = lambda)
+        end
+    end
+end
+
+end # module
