@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 17,
                     help="particles of the serial CPU FMM sample (0 = skip)")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -125,6 +126,24 @@ def main():
         res["fp64"] = {"value": a.N / ms_d, "ms_per_step": ms_d * 1e3,
                        "stages_ms": stage_avg(st_d),
                        "roofline": roofline(st_d[-1], stage_avg(st_d), a.n, "fp64", a.N)}
+    if not a.skip_fp64 and not a.skip_adjoint:
+        # reverse mode (FP64) of L = sum w (|E|^2 + |B|^2): cotangents 2 w E, 2 w B
+        w = torch.rand(a.N, 1, dtype=torch.float64, device=dev,
+                       generator=torch.Generator(device=dev).manual_seed(40))
+        Eb, Bb = 2 * w * bufs_d[2], 2 * w * bufs_d[3]
+        gpos, gmom = torch.empty_like(dpos), torch.empty_like(dpos)
+        call = lambda: ctx.field_vjp_device(a.N, dpos.data_ptr(), dmom.data_ptr(), Eb.data_ptr(),
+                                            Bb.data_ptr(), gpos.data_ptr(), gmom.data_ptr(),
+                                            -1.0, lam, a.n, a.N0, a.eta, "fp64")
+        for _ in range(a.warmup):
+            call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sa = [call() for _ in range(a.steps)]
+        torch.cuda.synchronize()
+        ms_a = (time.perf_counter() - t0) / a.steps
+        res["adjoint"] = {"ms_per_step": ms_a * 1e3, "ratio_to_fp64_forward": ms_a / ms_d,
+                          "stages_ms": stage_avg(sa)}
     # accuracy, outside the timed region
     Em, Bm = bufs_m[2].cpu().numpy(), bufs_m[3].cpu().numpy()
     acc = {}
@@ -154,6 +173,8 @@ def main():
         "stages_ms": stages,
         "roofline": roofline(s, stages, a.n, "mixed", a.N),
     }
+    if "adjoint" in res:
+        out["adjoint_fp64"] = res["adjoint"]
     if "fp64" in res:
         out["fp64"] = res["fp64"]
         out["speedup_vs_fp64"] = out["value"] / res["fp64"]["value"]

@@ -1,0 +1,1061 @@
+// adjoint.hip -- hand-written reverse mode (vector-Jacobian product) of the FMM
+// field map (x, p) -> (E, B) at fixed tree topology and interaction lists.
+// There is no counterpart in the reference; it differentiates the forward
+// operators restated from src/fmm/*.jl and src/gpu_fmm/*.jl.
+//
+// Forward quantities (unscaled, amp = 2.8179403699772166e-15 q / lambda applied last):
+//   K(R; p) = R d^-3/2,  d = R.R + (p.R)^2 + eps
+//   near:  E_i += g_j K(x_i - x_j; p_j),  B_i += p_j x K
+//   far:   multipoles M (P2M, M2M), locals L (M2L with p_avg, L2L), L2P
+// Cotangents o_i = amp (Ebar_i, Bbar_i).  For one kernel term with source
+// "charge" (g, m) and cotangent (e, b):  u = g e + b x m, and
+//   dl/dR = J^T u = d^-3/2 u - 3 d^-5/2 (R.u) (R + (p.R) p)
+//   dl/dp (through d) = -3 d^-5/2 (p.R) (R.u) R
+//   dl/dg = e.K,  dl/dm = K x b.
+// Interpolation weights w_a(x; X) (barycentric, last-match singular rule)
+// depend on the point x and on the cluster's nodes X_m = (c_m (b-a))/2 + (a+b)/2,
+// hence on the cluster's bounding box, hence on the particles that attain its
+// minimum and maximum.  For l = sum_a c_a w_a:
+//   dl/dx   = sum_a c_a w'_a,   w'_a = w_a (sum_k w_k/(x-X_k) - 1/(x-X_a))
+//   dl/dX_m = w_m/(x-X_m) (c_m - sum_a c_a w_a)
+// and at a node X_f (singular case): w'_a = (bw_a/bw_f)/(X_f-X_a) (a != f),
+// w'_f = -sum_{a!=f} w'_a, dl/dX_f = -dl/dx, dl/dX_m = 0 otherwise.
+// Order (reverse of the forward): L2P^T, L2L^T (bottom-up), M2L^T (target side
+// and source side over transposed lists), M2M^T (top-down), P2M^T, P2P^T (target
+// side and source side), then node-coordinate gradients -> bounding boxes ->
+// the extreme particles, and the mean-momentum gradient to every particle.
+// Every accumulation has one owner per output (no atomics): results are
+// bit-reproducible.  A self pair or coincident particles (R = 0) contribute
+// nothing (K = 0 identically).
+#include "fmm4rb_internal.hpp"
+
+namespace fmm4rb {
+
+template <int NP>
+__device__ inline double abary(int i) {
+    double w = (i & 1) ? -1.0 : 1.0;
+    if (i == 0) w = 0.5;
+    if (i == NP - 1) w = 0.5 * w;
+    return w;
+}
+
+// Weights w and derivatives dw = dw/dx of the NP nodes X at x; returns the
+// singular node (last within eps) or -1.
+template <int NP>
+__device__ inline int lgw_d(double x, const double *X, double *w, double *dw) {
+    int flag = -1;
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+        if (fabs(x - X[i]) < EPS64) flag = i;
+    if (flag >= 0) {
+        double s = 0.0;
+        const double bf = abary<NP>(flag);
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            w[i] = (i == flag) ? 1.0 : 0.0;
+            dw[i] = 0.0;
+            if (i != flag) {
+                dw[i] = (abary<NP>(i) / bf) / (X[flag] - X[i]);
+                s += dw[i];
+            }
+        }
+        dw[flag] = -s;
+        return flag;
+    }
+    double S = 0.0;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        w[i] = abary<NP>(i) / (x - X[i]);
+        S += w[i];
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        w[i] = w[i] / S;
+        t += w[i] / (x - X[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < NP; ++i) dw[i] = w[i] * (t - 1.0 / (x - X[i]));
+    return -1;
+}
+
+// Given l = sum_a c_a w_a(x; X): returns dl/dx and adds dl/dX_m to Xb[m].
+template <int NP>
+__device__ inline double axis_adj(double x, const double *X, const double *w, const double *dw,
+                                  int flag, const double *c, double *Xb) {
+    double xb = 0.0, cw = 0.0;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        xb += c[i] * dw[i];
+        cw += c[i] * w[i];
+    }
+    if (flag >= 0) {
+#pragma unroll
+        for (int m = 0; m < NP; ++m)
+            if (m == flag) Xb[m] -= xb;
+    } else {
+#pragma unroll
+        for (int m = 0; m < NP; ++m) Xb[m] += w[m] / (x - X[m]) * (c[m] - cw);
+    }
+    return xb;
+}
+
+constexpr int AT = 128;  // threads of the leaf / transfer adjoint kernels
+
+// Sum per-thread partials red[AT][W] into out[W] (+=), fixed order.
+template <int W>
+__device__ inline void block_sum_into(double (*red)[W], double *out) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < W; j += AT) {
+        double s = 0.0;
+        for (int t = 0; t < AT; ++t) s += red[t][j];
+        out[j] += s;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------- leaf adjoints
+// MODE 0 (L2P^T): A = forward locals (6 comps), o_i = cotangents ob[i];
+//   Lbar[leaf][a] = sum_i o_i W_i[a];  gx_i += d/dx sum_a (o_i.A[a]) W_i[a]; Xbar[leaf] += d/dX.
+// MODE 1 (P2M^T): A = Mbar (4 comps), o_i = (g_i, p_i);
+//   qbar_i = sum_a A[a] W_i[a] -> gp_i += qbar_p + qbar_g p_i/g_i;  gx_i, Xbar as above.
+// PART bit 1: the gradient (geometry) terms; bit 2: the Lbar values (MODE 0).
+template <int NP, int MODE, int PART>
+__global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ leaves,
+                                                 const int32_t *__restrict__ lo,
+                                                 const int32_t *__restrict__ cnt,
+                                                 const double *__restrict__ sp8,
+                                                 const double *__restrict__ ncoord,
+                                                 const double *__restrict__ A,
+                                                 const double *__restrict__ ob,
+                                                 double *__restrict__ Abar, double *__restrict__ gx,
+                                                 double *__restrict__ gp,
+                                                 double *__restrict__ Xbar) {
+    constexpr int NN = NP * NP * NP, NC = MODE == 0 ? 6 : 4, TPT = (NN + AT - 1) / AT;
+    __shared__ double nd[3 * NP];
+    __shared__ double red[AT][3 * NP];  // Xbar partials, then the weight tiles
+    __shared__ double Ot[AT][6];
+    double(*Wt)[3 * NP] = red;
+    const int tid = threadIdx.x;
+    const int32_t k = leaves[blockIdx.x];
+    const int32_t a = lo[k], c = cnt[k];
+    if (tid < 3 * NP) nd[tid] = ncoord[(int64_t)k * 3 * NP + tid];
+    double xbl[3 * NP];
+#pragma unroll
+    for (int j = 0; j < 3 * NP; ++j) xbl[j] = 0.0;
+    const double *Ak = A + (int64_t)k * NN * NC;
+    __syncthreads();
+    // per-particle gradient terms
+    for (int p = tid; (PART & 1) && p < c; p += AT) {
+        const int64_t i = a + p;
+        const double *x = sp8 + 8 * i;
+        double w[3][NP], dw[3][NP];
+        int fl[3];
+        for (int d = 0; d < 3; ++d) fl[d] = lgw_d<NP>(x[d], nd + d * NP, w[d], dw[d]);
+        double o[NC];
+        if (MODE == 0) {
+#pragma unroll
+            for (int m = 0; m < NC; ++m) o[m] = ob[6 * i + m];
+        } else {
+            o[0] = x[3];
+            o[1] = x[4];
+            o[2] = x[5];
+            o[3] = x[6];
+        }
+        double T[3][NP];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int m = 0; m < NP; ++m) T[d][m] = 0.0;
+        double qb[NC];
+#pragma unroll
+        for (int m = 0; m < NC; ++m) qb[m] = 0.0;
+        for (int kz = 0; kz < NP; ++kz)
+            for (int jy = 0; jy < NP; ++jy) {
+                const double wyz = w[1][jy] * w[2][kz];
+#pragma unroll
+                for (int ix = 0; ix < NP; ++ix) {
+                    const double *Av = Ak + (ix + NP * (jy + NP * kz)) * NC;
+                    double s = 0.0;
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) s += o[m] * Av[m];
+                    T[0][ix] += s * wyz;
+                    T[1][jy] += s * (w[0][ix] * w[2][kz]);
+                    T[2][kz] += s * (w[0][ix] * w[1][jy]);
+                    if (MODE == 1) {
+                        const double W3 = w[0][ix] * wyz;
+#pragma unroll
+                        for (int m = 0; m < NC; ++m) qb[m] += Av[m] * W3;
+                    }
+                }
+            }
+        double g[3];
+        for (int d = 0; d < 3; ++d)
+            g[d] = axis_adj<NP>(x[d], nd + d * NP, w[d], dw[d], fl[d], T[d], xbl + d * NP);
+        for (int d = 0; d < 3; ++d) gx[3 * i + d] += g[d];
+        if (MODE == 1) {
+            const double gi = qb[0] / x[3];
+            for (int d = 0; d < 3; ++d) gp[3 * i + d] += qb[1 + d] + gi * x[4 + d];
+        }
+    }
+    if (PART & 1) {
+#pragma unroll
+        for (int j = 0; j < 3 * NP; ++j) red[tid][j] = xbl[j];
+        block_sum_into<3 * NP>(red, Xbar + (int64_t)k * 3 * NP);
+    }
+    if (MODE == 0 && (PART & 2)) {  // Lbar[leaf][a] = sum_i o_i W_i[a]
+        double acc[TPT][6];
+#pragma unroll
+        for (int q = 0; q < TPT; ++q)
+#pragma unroll
+            for (int m = 0; m < 6; ++m) acc[q][m] = 0.0;
+        for (int p0 = 0; p0 < c; p0 += AT) {
+            __syncthreads();
+            const int p = p0 + tid;
+            if (p < c) {
+                const int64_t i = a + p;
+                const double *x = sp8 + 8 * i;
+                double dw[NP];
+                for (int d = 0; d < 3; ++d) lgw_d<NP>(x[d], nd + d * NP, &Wt[tid][d * NP], dw);
+                for (int m = 0; m < 6; ++m) Ot[tid][m] = ob[6 * i + m];
+            }
+            __syncthreads();
+            const int mm = min(AT, c - p0);
+#pragma unroll
+            for (int q = 0; q < TPT; ++q) {
+                const int node = tid + q * AT;
+                if (node < NN) {
+                    const int ix = node % NP, jy = (node / NP) % NP, kz = node / (NP * NP);
+                    for (int pp = 0; pp < mm; ++pp) {
+                        const double W3 = (Wt[pp][ix] * Wt[pp][NP + jy]) * Wt[pp][2 * NP + kz];
+#pragma unroll
+                        for (int m = 0; m < 6; ++m) acc[q][m] += Ot[pp][m] * W3;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            const int node = tid + q * AT;
+            if (node < NN)
+#pragma unroll
+                for (int m = 0; m < 6; ++m) Abar[((int64_t)k * NN + node) * 6 + m] = acc[q][m];
+        }
+    }
+}
+
+// ------------------------------------------------------- transfer adjoints
+// Per parent P with children c = 2P+1, 2P+2; Om_c(a,b) = prod_d w_{P,d,b_d}(X_{c,d,a_d}).
+// MODE 0 (L2L^T): Lbar_P[b] = sum_c sum_a Om_c(a,b) Lbar_c[a]; geometry with
+//                 U_c = Lbar_c (child side), V = L_P (forward parent locals), 6 comps.
+// MODE 1 (M2M^T): Mbar_c[a] += sum_b Om_c(a,b) Mbar_P[b]; geometry with
+//                 U_c = M_c (forward child multipoles), V = Mbar_P, 4 comps.
+// Geometry: l = sum_c sum_{a,b} (U_c[a].V[b]) Om_c(a,b); dl/dOm_x[ax][bx] =
+// sum_{ay,az} U[a] . (sum_{by,bz} V[b] Om_y[ay][by] Om_z[az][bz]), likewise y, z.
+// PART bit 1: geometry (Xbar); bit 2: the value propagation.
+template <int NP, int MODE, int PART>
+__global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__ parents,
+                                                     const double *__restrict__ ncoord,
+                                                     const double *__restrict__ Uarr,
+                                                     const double *__restrict__ Varr,
+                                                     double *__restrict__ out,
+                                                     double *__restrict__ Xbar) {
+    constexpr int NN = NP * NP * NP, NC = MODE == 0 ? 6 : 4, TPT = (NN + AT - 1) / AT;
+    __shared__ double Om[3][NP][NP], dOm[3][NP][NP];  // [axis][child node a][parent node b]
+    __shared__ int fl[3][NP];
+    __shared__ double pn[3 * NP], cn[3 * NP];
+    __shared__ double bU[NN], bV[NN], t1[NN], t2[NN];
+    __shared__ double red[AT][3 * NP];
+    const int tid = threadIdx.x;
+    const int32_t P = parents[blockIdx.x];
+    if (tid < 3 * NP) pn[tid] = ncoord[(int64_t)P * 3 * NP + tid];
+    double xbP[3 * NP];  // this thread's share of dl/dX_P
+#pragma unroll
+    for (int j = 0; j < 3 * NP; ++j) xbP[j] = 0.0;
+    double accP[TPT][NC];  // MODE 0: Lbar_P
+#pragma unroll
+    for (int q = 0; q < TPT; ++q)
+#pragma unroll
+        for (int m = 0; m < NC; ++m) accP[q][m] = 0.0;
+    for (int ch = 0; ch < 2; ++ch) {
+        const int32_t C = 2 * P + 1 + ch;
+        __syncthreads();
+        if (tid < 3 * NP) cn[tid] = ncoord[(int64_t)C * 3 * NP + tid];
+        __syncthreads();
+        if (tid < 3 * NP) {
+            const int d = tid / NP, ai = tid % NP;
+            fl[d][ai] = lgw_d<NP>(cn[tid], pn + d * NP, &Om[d][ai][0], &dOm[d][ai][0]);
+        }
+        // dl/dOm accumulated by the thread owning the (a_d, b_d) pair
+        double gO[3] = {0.0, 0.0, 0.0};
+        const int pa = tid / NP, pb = tid % NP;
+        const bool own = tid < NP * NP;
+        double accC[TPT][NC];  // MODE 1: Mbar_c increment
+#pragma unroll
+        for (int q = 0; q < TPT; ++q)
+#pragma unroll
+            for (int m = 0; m < NC; ++m) accC[q][m] = 0.0;
+        for (int comp = 0; comp < NC; ++comp) {
+            __syncthreads();
+            for (int e = tid; e < NN; e += AT) {
+                bU[e] = MODE == 0 ? Uarr[((int64_t)C * NN + e) * NC + comp]
+                                  : Uarr[((int64_t)C * NN + e) * NC + comp];
+                bV[e] = Varr[((int64_t)P * NN + e) * NC + comp];
+            }
+            __syncthreads();
+            // ---- value propagation
+            if (!(PART & 2)) {
+            } else if (MODE == 0) {  // Lbar_P[b] += sum_a Om(a,b) Lbar_c[a]: contract a_x, a_y, a_z
+                for (int e = tid; e < NN; e += AT) {  // t1[bx, ay, az]
+                    const int bx = e % NP, r = e / NP;
+                    double s = 0.0;
+                    for (int ax = 0; ax < NP; ++ax) s += Om[0][ax][bx] * bU[ax + NP * r];
+                    t1[e] = s;
+                }
+                __syncthreads();
+                for (int e = tid; e < NN; e += AT) {  // t2[bx, by, az]
+                    const int bx = e % NP, by = (e / NP) % NP, az = e / (NP * NP);
+                    double s = 0.0;
+                    for (int ay = 0; ay < NP; ++ay) s += Om[1][ay][by] * t1[bx + NP * (ay + NP * az)];
+                    t2[e] = s;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < TPT; ++q) {
+                    const int e = tid + q * AT;
+                    if (e < NN) {
+                        const int ij = e % (NP * NP), bz = e / (NP * NP);
+                        double s = 0.0;
+                        for (int az = 0; az < NP; ++az) s += Om[2][az][bz] * t2[ij + NP * NP * az];
+                        accP[q][comp] += s;
+                    }
+                }
+            } else {  // Mbar_c[a] += sum_b Om(a,b) Mbar_P[b]: contract b_x, b_y, b_z
+                for (int e = tid; e < NN; e += AT) {  // t1[ax, by, bz]
+                    const int ax = e % NP, r = e / NP;
+                    double s = 0.0;
+                    for (int bx = 0; bx < NP; ++bx) s += Om[0][ax][bx] * bV[bx + NP * r];
+                    t1[e] = s;
+                }
+                __syncthreads();
+                for (int e = tid; e < NN; e += AT) {  // t2[ax, ay, bz]
+                    const int ax = e % NP, ay = (e / NP) % NP, bz = e / (NP * NP);
+                    double s = 0.0;
+                    for (int by = 0; by < NP; ++by) s += Om[1][ay][by] * t1[ax + NP * (by + NP * bz)];
+                    t2[e] = s;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < TPT; ++q) {
+                    const int e = tid + q * AT;
+                    if (e < NN) {
+                        const int ij = e % (NP * NP), az = e / (NP * NP);
+                        double s = 0.0;
+                        for (int bz = 0; bz < NP; ++bz) s += Om[2][az][bz] * t2[ij + NP * NP * bz];
+                        accC[q][comp] += s;
+                    }
+                }
+            }
+            // ---- geometry: three axes
+            for (int ax3 = 0; (PART & 1) && ax3 < 3; ++ax3) {
+                // V'[..] = V contracted over the two other axes with Om (b -> a)
+                const int d1 = (ax3 + 1) % 3, d2 = (ax3 + 2) % 3;
+                __syncthreads();
+                for (int e = tid; e < NN; e += AT) {  // contract axis d1: index b_d1 -> a_d1
+                    int id[3] = {e % NP, (e / NP) % NP, e / (NP * NP)};
+                    const int a1 = id[d1];
+                    double s = 0.0;
+                    for (int b1 = 0; b1 < NP; ++b1) {
+                        id[d1] = b1;
+                        s += Om[d1][a1][b1] * bV[id[0] + NP * (id[1] + NP * id[2])];
+                    }
+                    t1[e] = s;
+                }
+                __syncthreads();
+                for (int e = tid; e < NN; e += AT) {  // contract axis d2
+                    int id[3] = {e % NP, (e / NP) % NP, e / (NP * NP)};
+                    const int a2 = id[d2];
+                    double s = 0.0;
+                    for (int b2 = 0; b2 < NP; ++b2) {
+                        id[d2] = b2;
+                        s += Om[d2][a2][b2] * t1[id[0] + NP * (id[1] + NP * id[2])];
+                    }
+                    t2[e] = s;
+                }
+                __syncthreads();
+                if (own) {  // gO[ax3] += sum over the other a indices of U[a] t2[(a with a_ax3 -> b)]
+                    double s = 0.0;
+                    for (int u1 = 0; u1 < NP; ++u1)
+                        for (int u2 = 0; u2 < NP; ++u2) {
+                            int ia[3], ib[3];
+                            ia[ax3] = pa;
+                            ib[ax3] = pb;
+                            ia[d1] = ib[d1] = u1;
+                            ia[d2] = ib[d2] = u2;
+                            s += bU[ia[0] + NP * (ia[1] + NP * ia[2])] *
+                                 t2[ib[0] + NP * (ib[1] + NP * ib[2])];
+                        }
+                    gO[ax3] += s;
+                }
+            }
+        }
+        // child's value output (MODE 1)
+        if (MODE == 1 && (PART & 2)) {
+#pragma unroll
+            for (int q = 0; q < TPT; ++q) {
+                const int e = tid + q * AT;
+                if (e < NN)
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) out[((int64_t)C * NN + e) * NC + m] += accC[q][m];
+            }
+        }
+        // dl/dOm -> dl/dX_c (evaluation points) and dl/dX_P (nodes)
+        __syncthreads();
+        if ((PART & 1) && own)
+            for (int d = 0; d < 3; ++d) red[tid][d] = gO[d];  // red[(a,b)][axis]
+        __syncthreads();
+        if ((PART & 1) && tid < 3 * NP) {
+            const int d = tid / NP, ai = tid % NP;
+            double cvec[NP];
+            for (int b = 0; b < NP; ++b) cvec[b] = red[ai * NP + b][d];
+            double xP[NP];
+            for (int m = 0; m < NP; ++m) xP[m] = 0.0;
+            const double gxc = axis_adj<NP>(cn[tid], pn + d * NP, &Om[d][ai][0], &dOm[d][ai][0],
+                                            fl[d][ai], cvec, xP);
+            Xbar[(int64_t)C * 3 * NP + tid] += gxc;
+            // stash dl/dX_P of this child row in red[...][3 + ...] below
+            for (int m = 0; m < NP; ++m) xbP[d * NP + m] += xP[m];
+        }
+    }
+    if (MODE == 0 && (PART & 2)) {
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            const int e = tid + q * AT;
+            if (e < NN)
+#pragma unroll
+                for (int m = 0; m < NC; ++m) out[((int64_t)P * NN + e) * NC + m] = accP[q][m];
+        }
+    }
+    if (PART & 1) {
+#pragma unroll
+        for (int j = 0; j < 3 * NP; ++j) red[tid][j] = xbP[j];
+        block_sum_into<3 * NP>(red, Xbar + (int64_t)P * 3 * NP);
+    }
+}
+
+// ------------------------------------------------------------- M2L adjoints
+template <int NP>
+struct AShape {
+    static constexpr int NN = NP * NP * NP;
+    static constexpr int BLK = NN >= 1024 ? 1024 : ((NN + 63) / 64) * 64;
+    static constexpr int TPT = (NN + BLK - 1) / BLK;
+};
+
+// Target side: per target node a, over its sources: dl/dY_t[a] and dl/dp_avg;
+// with FWD the same pass also forms the forward M2L locals (row sums as in
+// k_m2l) and stores them in loc.
+template <int NP, bool FWD>
+__global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
+    const int32_t *__restrict__ off, const uint64_t *__restrict__ keys, int sb,
+    const double *__restrict__ ncoord, const double *__restrict__ mp4,
+    const double *__restrict__ lbar, const double *__restrict__ pavg4, double *__restrict__ Xbar,
+    double *__restrict__ pbar_part, double *__restrict__ loc) {
+    constexpr int NN = AShape<NP>::NN, BLK = AShape<NP>::BLK, TPT = AShape<NP>::TPT;
+    constexpr int SH = (4 * NN > 3 * BLK) ? 4 * NN : 3 * BLK;
+    __shared__ double shb[SH];
+    __shared__ double sn[3 * NP];
+    double4 *smp = reinterpret_cast<double4 *>(shb);
+    double(*yb)[3] = reinterpret_cast<double(*)[3]>(shb);
+    double(*pr3)[3] = reinterpret_cast<double(*)[3]>(shb);
+    const int tid = threadIdx.x;
+    const int32_t t = blockIdx.x;
+    const int32_t beg = off[t], end = off[t + 1];
+    const uint64_t mask = (1ull << sb) - 1ull;
+    const double px = pavg4[0], py = pavg4[1], pz = pavg4[2];
+    double tx[TPT], ty[TPT], tz[TPT], e[TPT][3], b[TPT][3], Yb[TPT][3], F[TPT][6];
+    double pb[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        const int node = min(tid + q * BLK, NN - 1);
+        tx[q] = ncoord[((int64_t)t * 3 + 0) * NP + node % NP];
+        ty[q] = ncoord[((int64_t)t * 3 + 1) * NP + (node / NP) % NP];
+        tz[q] = ncoord[((int64_t)t * 3 + 2) * NP + node / (NP * NP)];
+        const double *lb = lbar + ((int64_t)t * NN + node) * 6;
+        for (int d = 0; d < 3; ++d) {
+            e[q][d] = lb[d];
+            b[q][d] = lb[3 + d];
+            Yb[q][d] = 0.0;
+        }
+        for (int d = 0; d < 6; ++d) F[q][d] = 0.0;
+    }
+    for (int32_t ei = beg; ei < end; ++ei) {
+        const int64_t s = (int64_t)(keys[ei] & mask);
+        __syncthreads();
+        for (int j = tid; j < NN; j += BLK) smp[j] = reinterpret_cast<const double4 *>(mp4)[s * NN + j];
+        if (tid < 3 * NP) sn[tid] = ncoord[s * 3 * NP + tid];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            if (tid + q * BLK >= NN) continue;
+            for (int sk = 0; sk < NP; ++sk) {
+                const double dz = tz[q] - sn[2 * NP + sk];
+                for (int sj = 0; sj < NP; ++sj) {
+                    const double dy = ty[q] - sn[NP + sj];
+                    const double rrjk = fma(dy, dy, dz * dz) + EPS64;
+                    const double prjk = fma(py, dy, pz * dz);
+                    double G = 0.0, H = 0.0, Ax = 0.0, Ay = 0.0, Az = 0.0, Uy = 0.0, Uz = 0.0;
+#pragma unroll
+                    for (int si = 0; si < NP; ++si) {
+                        const double dx = tx[q] - sn[si];
+                        const double pr = fma(px, dx, prjk);
+                        const double d = fma(pr, pr, fma(dx, dx, rrjk));
+                        const double y = rsqrt_nr(d);
+                        const double y2 = y * y, y3 = y2 * y, y5 = y3 * y2;
+                        const double4 m = smp[si + NP * (sj + NP * sk)];
+                        if (FWD) {  // forward row sums (see k_m2l)
+                            const double uu = y3 * dx;
+                            G = fma(y3, m.x, G);
+                            H = fma(uu, m.x, H);
+                            Ax = fma(y3, m.y, Ax);
+                            Ay = fma(y3, m.z, Ay);
+                            Az = fma(y3, m.w, Az);
+                            Uy = fma(uu, m.z, Uy);
+                            Uz = fma(uu, m.w, Uz);
+                        }
+                        const double u0 = fma(m.x, e[q][0], b[q][1] * m.w - b[q][2] * m.z);
+                        const double u1 = fma(m.x, e[q][1], b[q][2] * m.y - b[q][0] * m.w);
+                        const double u2 = fma(m.x, e[q][2], b[q][0] * m.z - b[q][1] * m.y);
+                        const double Ru = fma(dx, u0, fma(dy, u1, dz * u2));
+                        const double c3 = 3.0 * y5 * Ru;
+                        Yb[q][0] += y3 * u0 - c3 * fma(pr, px, dx);
+                        Yb[q][1] += y3 * u1 - c3 * fma(pr, py, dy);
+                        Yb[q][2] += y3 * u2 - c3 * fma(pr, pz, dz);
+                        const double cp = -c3 * pr;
+                        pb[0] = fma(cp, dx, pb[0]);
+                        pb[1] = fma(cp, dy, pb[1]);
+                        pb[2] = fma(cp, dz, pb[2]);
+                    }
+                    if (FWD) {
+                        F[q][0] += H;
+                        F[q][1] = fma(dy, G, F[q][1]);
+                        F[q][2] = fma(dz, G, F[q][2]);
+                        F[q][3] += dz * Ay - dy * Az;
+                        F[q][4] += Uz - dz * Ax;
+                        F[q][5] += dy * Ax - Uy;
+                    }
+                }
+            }
+        }
+    }
+    if (FWD) {
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            const int node = tid + q * BLK;
+            if (node < NN)
+                for (int d = 0; d < 6; ++d) loc[((int64_t)t * NN + node) * 6 + d] = F[q][d];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        const int node = tid + q * BLK;
+        if (node < NN)
+            for (int d = 0; d < 3; ++d) yb[node][d] = Yb[q][d];
+    }
+    __syncthreads();
+    if (tid < 3 * NP) {  // dl/dX_{t,d,m} = sum over nodes with index m on axis d
+        const int d = tid / NP, mi = tid % NP;
+        double s = 0.0;
+        for (int u1 = 0; u1 < NP; ++u1)
+            for (int u2 = 0; u2 < NP; ++u2) {
+                int id[3];
+                id[d] = mi;
+                id[(d + 1) % 3] = u1;
+                id[(d + 2) % 3] = u2;
+                s += yb[id[0] + NP * (id[1] + NP * id[2])][d];
+            }
+        Xbar[(int64_t)t * 3 * NP + tid] += s;
+    }
+    __syncthreads();
+    for (int d = 0; d < 3; ++d) pr3[tid][d] = pb[d];
+    __syncthreads();
+    if (tid < 3) {
+        double s = 0.0;
+        for (int j = 0; j < BLK; ++j) s += pr3[j][tid];
+        pbar_part[3 * (int64_t)t + tid] = s;
+    }
+}
+
+// Source side: per source node b, over the targets that list it: Mbar_s[b]
+// (= dl/dgamma, dl/dm) and dl/dY_s[b].
+template <int NP>
+__global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
+    const int32_t *__restrict__ offT, const uint64_t *__restrict__ keysT, int sb,
+    const double *__restrict__ ncoord, const double *__restrict__ mp4,
+    const double *__restrict__ lbar, const double *__restrict__ pavg4, double *__restrict__ mbar,
+    double *__restrict__ Xbar) {
+    constexpr int NN = AShape<NP>::NN, BLK = AShape<NP>::BLK, TPT = AShape<NP>::TPT;
+    __shared__ double slb[NN * 6];
+    __shared__ double tn[3 * NP];
+    double(*yb)[3] = reinterpret_cast<double(*)[3]>(slb);
+    const int tid = threadIdx.x;
+    const int32_t s = blockIdx.x;
+    const int32_t beg = offT[s], end = offT[s + 1];
+    const uint64_t mask = (1ull << sb) - 1ull;
+    const double px = pavg4[0], py = pavg4[1], pz = pavg4[2];
+    double sx[TPT], sy[TPT], sz[TPT], Yb[TPT][3], gb[TPT][4];
+    double4 mq[TPT];
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        const int node = min(tid + q * BLK, NN - 1);
+        sx[q] = ncoord[((int64_t)s * 3 + 0) * NP + node % NP];
+        sy[q] = ncoord[((int64_t)s * 3 + 1) * NP + (node / NP) % NP];
+        sz[q] = ncoord[((int64_t)s * 3 + 2) * NP + node / (NP * NP)];
+        mq[q] = reinterpret_cast<const double4 *>(mp4)[(int64_t)s * NN + node];
+        for (int d = 0; d < 3; ++d) Yb[q][d] = 0.0;
+        for (int d = 0; d < 4; ++d) gb[q][d] = 0.0;
+    }
+    for (int32_t ei = beg; ei < end; ++ei) {
+        const int64_t t = (int64_t)(keysT[ei] & mask);
+        __syncthreads();
+        for (int j = tid; j < NN * 6; j += BLK) slb[j] = lbar[t * NN * 6 + j];
+        if (tid < 3 * NP) tn[tid] = ncoord[t * 3 * NP + tid];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            if (tid + q * BLK >= NN) continue;
+            const double4 m = mq[q];
+            for (int tk = 0; tk < NP; ++tk) {
+                const double dz = tn[2 * NP + tk] - sz[q];
+                for (int tj = 0; tj < NP; ++tj) {
+                    const double dy = tn[NP + tj] - sy[q];
+                    const double rrjk = fma(dy, dy, dz * dz) + EPS64;
+                    const double prjk = fma(py, dy, pz * dz);
+#pragma unroll
+                    for (int ti = 0; ti < NP; ++ti) {
+                        const double dx = tn[ti] - sx[q];
+                        const double pr = fma(px, dx, prjk);
+                        const double d = fma(pr, pr, fma(dx, dx, rrjk));
+                        const double y = rsqrt_nr(d);
+                        const double y2 = y * y, y3 = y2 * y, y5 = y3 * y2;
+                        const double *o = slb + 6 * (ti + NP * (tj + NP * tk));
+                        const double e0 = o[0], e1 = o[1], e2 = o[2], b0 = o[3], b1 = o[4], b2 = o[5];
+                        // dl/dgamma = e.K, dl/dm = K x b
+                        gb[q][0] += y3 * fma(e0, dx, fma(e1, dy, e2 * dz));
+                        gb[q][1] += y3 * (dy * b2 - dz * b1);
+                        gb[q][2] += y3 * (dz * b0 - dx * b2);
+                        gb[q][3] += y3 * (dx * b1 - dy * b0);
+                        const double u0 = fma(m.x, e0, b1 * m.w - b2 * m.z);
+                        const double u1 = fma(m.x, e1, b2 * m.y - b0 * m.w);
+                        const double u2 = fma(m.x, e2, b0 * m.z - b1 * m.y);
+                        const double Ru = fma(dx, u0, fma(dy, u1, dz * u2));
+                        const double c3 = 3.0 * y5 * Ru;
+                        Yb[q][0] -= y3 * u0 - c3 * fma(pr, px, dx);
+                        Yb[q][1] -= y3 * u1 - c3 * fma(pr, py, dy);
+                        Yb[q][2] -= y3 * u2 - c3 * fma(pr, pz, dz);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        const int node = tid + q * BLK;
+        if (node < NN) {
+            double *o = mbar + ((int64_t)s * NN + node) * 4;
+            for (int d = 0; d < 4; ++d) o[d] = gb[q][d];
+            for (int d = 0; d < 3; ++d) yb[node][d] = Yb[q][d];
+        }
+    }
+    __syncthreads();
+    if (tid < 3 * NP) {
+        const int d = tid / NP, mi = tid % NP;
+        double sum = 0.0;
+        for (int u1 = 0; u1 < NP; ++u1)
+            for (int u2 = 0; u2 < NP; ++u2) {
+                int id[3];
+                id[d] = mi;
+                id[(d + 1) % 3] = u1;
+                id[(d + 2) % 3] = u2;
+                sum += yb[id[0] + NP * (id[1] + NP * id[2])][d];
+            }
+        Xbar[(int64_t)s * 3 * NP + tid] += sum;
+    }
+}
+
+// ------------------------------------------------------------- P2P adjoints
+constexpr int PT = 256;
+
+// Target side: gx_i += sum_j J^T u_ij.
+__global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ leaves,
+                                                  const int32_t *__restrict__ off,
+                                                  const uint64_t *__restrict__ keys, int sb,
+                                                  const int32_t *__restrict__ lo,
+                                                  const int32_t *__restrict__ cnt,
+                                                  const double *__restrict__ sp8,
+                                                  const double *__restrict__ ob,
+                                                  double *__restrict__ gx) {
+    __shared__ double4 sx[PT], sq[PT];
+    const int BT = blockDim.x, tid = threadIdx.x;
+    const int32_t t = leaves[blockIdx.x];
+    const int32_t a = lo[t], ct = cnt[t];
+    const int32_t beg = off[t], end = off[t + 1];
+    const uint64_t mask = (1ull << sb) - 1ull;
+    const double4 *p8 = reinterpret_cast<const double4 *>(sp8);
+    for (int tb = 0; tb < ct; tb += BT) {
+        const int i = tb + tid;
+        const bool act = i < ct;
+        const int64_t ii = a + (act ? i : 0);
+        const double4 xi = p8[2 * ii];
+        double e[3], b[3];
+        for (int d = 0; d < 3; ++d) {
+            e[d] = ob[6 * ii + d];
+            b[d] = ob[6 * ii + 3 + d];
+        }
+        double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+        for (int32_t ei = beg; ei < end; ++ei) {
+            const int32_t s = (int32_t)(keys[ei] & mask);
+            const int32_t sa = lo[s], cs = cnt[s];
+            for (int s0 = 0; s0 < cs; s0 += BT) {
+                __syncthreads();
+                const int j = s0 + tid;
+                if (j < cs) {
+                    const double4 x4 = p8[2 * (int64_t)(sa + j)];
+                    const double4 p4 = p8[2 * (int64_t)(sa + j) + 1];
+                    sx[tid] = x4;
+                    sq[tid] = make_double4(x4.w, p4.x, p4.y, p4.z);
+                }
+                __syncthreads();
+                const int m = min(BT, cs - s0);
+                if (act)
+                    for (int jj = 0; jj < m; ++jj) {
+                        const double4 xs = sx[jj], q = sq[jj];
+                        const double R0 = xi.x - xs.x, R1 = xi.y - xs.y, R2 = xi.z - xs.z;
+                        if (R0 == 0.0 && R1 == 0.0 && R2 == 0.0) continue;
+                        const double pr = fma(q.y, R0, fma(q.z, R1, q.w * R2));
+                        const double d = fma(pr, pr, fma(R0, R0, fma(R1, R1, R2 * R2))) + EPS64;
+                        const double y = rsqrt_nr(d);
+                        const double y2 = y * y, y3 = y2 * y, y5 = y3 * y2;
+                        const double u0 = fma(q.x, e[0], b[1] * q.w - b[2] * q.z);
+                        const double u1 = fma(q.x, e[1], b[2] * q.y - b[0] * q.w);
+                        const double u2 = fma(q.x, e[2], b[0] * q.z - b[1] * q.y);
+                        const double Ru = fma(R0, u0, fma(R1, u1, R2 * u2));
+                        const double c3 = 3.0 * y5 * Ru;
+                        g0 += y3 * u0 - c3 * fma(pr, q.y, R0);
+                        g1 += y3 * u1 - c3 * fma(pr, q.z, R1);
+                        g2 += y3 * u2 - c3 * fma(pr, q.w, R2);
+                    }
+            }
+        }
+        if (act) {
+            gx[3 * ii] += g0;
+            gx[3 * ii + 1] += g1;
+            gx[3 * ii + 2] += g2;
+        }
+    }
+}
+
+// Source side: gx_j -= sum_i J^T u_ij;  gp_j += (e.K) p/g + K x b + dK/dp^T u.
+__global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ leaves,
+                                                  const int32_t *__restrict__ offT,
+                                                  const uint64_t *__restrict__ keysT, int sb,
+                                                  const int32_t *__restrict__ lo,
+                                                  const int32_t *__restrict__ cnt,
+                                                  const double *__restrict__ sp8,
+                                                  const double *__restrict__ ob,
+                                                  double *__restrict__ gx, double *__restrict__ gp) {
+    __shared__ double4 tx[PT], te[PT], tb4[PT];
+    const int BT = blockDim.x, tid = threadIdx.x;
+    const int32_t s = leaves[blockIdx.x];
+    const int32_t a = lo[s], cs = cnt[s];
+    const int32_t beg = offT[s], end = offT[s + 1];
+    const uint64_t mask = (1ull << sb) - 1ull;
+    const double4 *p8 = reinterpret_cast<const double4 *>(sp8);
+    for (int sb0 = 0; sb0 < cs; sb0 += BT) {
+        const int j = sb0 + tid;
+        const bool act = j < cs;
+        const int64_t jj = a + (act ? j : 0);
+        const double4 xs = p8[2 * jj];
+        const double4 p4 = p8[2 * jj + 1];
+        const double g = xs.w, px = p4.x, py = p4.y, pz = p4.z;
+        double gx0 = 0.0, gx1 = 0.0, gx2 = 0.0, gp0 = 0.0, gp1 = 0.0, gp2 = 0.0;
+        for (int32_t ei = beg; ei < end; ++ei) {
+            const int32_t t = (int32_t)(keysT[ei] & mask);
+            const int32_t ta = lo[t], ct = cnt[t];
+            for (int t0 = 0; t0 < ct; t0 += BT) {
+                __syncthreads();
+                const int i = t0 + tid;
+                if (i < ct) {
+                    const int64_t ii = ta + i;
+                    tx[tid] = p8[2 * ii];
+                    te[tid] = make_double4(ob[6 * ii], ob[6 * ii + 1], ob[6 * ii + 2], 0.0);
+                    tb4[tid] = make_double4(ob[6 * ii + 3], ob[6 * ii + 4], ob[6 * ii + 5], 0.0);
+                }
+                __syncthreads();
+                const int m = min(BT, ct - t0);
+                if (act)
+                    for (int k = 0; k < m; ++k) {
+                        const double4 xi = tx[k], e = te[k], b = tb4[k];
+                        const double R0 = xi.x - xs.x, R1 = xi.y - xs.y, R2 = xi.z - xs.z;
+                        if (R0 == 0.0 && R1 == 0.0 && R2 == 0.0) continue;
+                        const double pr = fma(px, R0, fma(py, R1, pz * R2));
+                        const double d = fma(pr, pr, fma(R0, R0, fma(R1, R1, R2 * R2))) + EPS64;
+                        const double y = rsqrt_nr(d);
+                        const double y2 = y * y, y3 = y2 * y, y5 = y3 * y2;
+                        const double u0 = fma(g, e.x, b.y * pz - b.z * py);
+                        const double u1 = fma(g, e.y, b.z * px - b.x * pz);
+                        const double u2 = fma(g, e.z, b.x * py - b.y * px);
+                        const double Ru = fma(R0, u0, fma(R1, u1, R2 * u2));
+                        const double c3 = 3.0 * y5 * Ru;
+                        gx0 -= y3 * u0 - c3 * fma(pr, px, R0);
+                        gx1 -= y3 * u1 - c3 * fma(pr, py, R1);
+                        gx2 -= y3 * u2 - c3 * fma(pr, pz, R2);
+                        const double eK = y3 * fma(e.x, R0, fma(e.y, R1, e.z * R2));
+                        const double cg = eK / g, cp = -c3 * pr;
+                        gp0 += fma(cg, px, fma(cp, R0, y3 * (R1 * b.z - R2 * b.y)));
+                        gp1 += fma(cg, py, fma(cp, R1, y3 * (R2 * b.x - R0 * b.z)));
+                        gp2 += fma(cg, pz, fma(cp, R2, y3 * (R0 * b.y - R1 * b.x)));
+                    }
+            }
+        }
+        if (act) {
+            gx[3 * jj] += gx0;
+            gx[3 * jj + 1] += gx1;
+            gx[3 * jj + 2] += gx2;
+            gp[3 * jj] += gp0;
+            gp[3 * jj + 1] += gp1;
+            gp[3 * jj + 2] += gp2;
+        }
+    }
+}
+
+// ---------------------------------------------------- bookkeeping kernels
+__global__ void k_adj_prep(int64_t N, const int32_t *__restrict__ parind,
+                           const double *__restrict__ Ebar, const double *__restrict__ Bbar,
+                           double amp, double *__restrict__ ob, double *__restrict__ gx,
+                           double *__restrict__ gp) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int64_t q = parind[i];
+    for (int d = 0; d < 3; ++d) {
+        ob[6 * i + d] = amp * Ebar[3 * q + d];
+        ob[6 * i + 3 + d] = amp * Bbar[3 * q + d];
+        gx[3 * i + d] = 0.0;
+        gp[3 * i + d] = 0.0;
+    }
+}
+
+__global__ void k_fill0(double *__restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// dl/d(bbox) per node: a_d (min) and b_d (max) through X_m = (c_m (b-a))/2 + (a+b)/2.
+__global__ void k_adj_bbox(int64_t nc, int np, const uint8_t *__restrict__ flags,
+                           const double *__restrict__ Xbar, const double *__restrict__ ctab,
+                           double *__restrict__ abar) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    for (int d = 0; d < 3; ++d) {
+        double sa = 0.0, sbm = 0.0;
+        if (flags[k] & 1)
+            for (int m = 0; m < np; ++m) {
+                const double xb = Xbar[(k * 3 + d) * np + m];
+                sa += xb * ((1.0 - ctab[m]) / 2.0);
+                sbm += xb * ((1.0 + ctab[m]) / 2.0);
+            }
+        abar[6 * k + d] = sa;
+        abar[6 * k + 3 + d] = sbm;
+    }
+}
+
+// Each particle collects the bbox gradients of the ancestors where it is the
+// argmin / argmax, then the mean-momentum gradient; results go to the caller's
+// original particle order.
+__global__ void k_adj_finish(int64_t N, const int32_t *__restrict__ parind,
+                             const int32_t *__restrict__ leafof, const int32_t *__restrict__ ext,
+                             const double *__restrict__ abar, const double *__restrict__ pbar,
+                             const double *__restrict__ gx, const double *__restrict__ gp,
+                             double *__restrict__ gpos, double *__restrict__ gmom) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int32_t p = parind[i];
+    double g[3] = {gx[3 * i], gx[3 * i + 1], gx[3 * i + 2]};
+    int32_t k = leafof[i];
+    for (;;) {
+        for (int qd = 0; qd < 6; ++qd)
+            if (ext[6 * (int64_t)k + qd] == p) g[qd % 3] += abar[6 * (int64_t)k + qd];
+        if (k == 0) break;
+        k = (k - 1) / 2;
+    }
+    for (int d = 0; d < 3; ++d) {
+        gpos[3 * (int64_t)p + d] = g[d];
+        gmom[3 * (int64_t)p + d] = gp[3 * i + d] + pbar[d] / (double)N;
+    }
+}
+
+// Fixed-order sum of the per-target mean-momentum partials.
+__global__ __launch_bounds__(256) void k_adj_pbar(const double *__restrict__ part, int64_t nc,
+                                                  double *__restrict__ out) {
+    __shared__ double s[3][256];
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int64_t k = threadIdx.x; k < nc; k += 256)
+        for (int d = 0; d < 3; ++d) a[d] += part[3 * k + d];
+    for (int d = 0; d < 3; ++d) s[d][threadIdx.x] = a[d];
+    for (int o = 128; o > 0; o >>= 1) {
+        __syncthreads();
+        if (threadIdx.x < (unsigned)o)
+            for (int d = 0; d < 3; ++d) s[d][threadIdx.x] += s[d][threadIdx.x + o];
+    }
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) out[d] = s[d][0];
+}
+
+// ----------------------------------------------------------------- driver
+#define ADJ_NP_SWITCH(np, F)                            \
+    switch (np) {                                       \
+        case 2: F(2); break;                            \
+        case 3: F(3); break;                            \
+        case 4: F(4); break;                            \
+        case 5: F(5); break;                            \
+        case 6: F(6); break;                            \
+        case 7: F(7); break;                            \
+        case 8: F(8); break;                            \
+        case 9: F(9); break;                            \
+        case 10: F(10); break;                          \
+        case 11: F(11); break;                          \
+        default: fail(FMM4RB_EINVAL, "n out of range"); \
+    }
+
+static inline unsigned nbk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+void transpose_lists(Ctx &c, const uint64_t *sorted, int64_t n, DevBuf &kT, DevBuf &kT_alt,
+                     DevBuf &offT, uint64_t **sortedT);
+
+void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, double *gpos,
+                 double *gmom) {
+    const Shape &S = c.shape;
+    hipStream_t st = c.stream;
+    const int64_t N = c.N, nc = S.nc, NN = c.nn, np = c.np;
+    const unsigned nleaf = (unsigned)S.leaves.size();
+    c.d_ob.ensure(6 * (size_t)N * 8);
+    c.d_gx.ensure(3 * (size_t)N * 8);
+    c.d_gp.ensure(3 * (size_t)N * 8);
+    c.d_lbar.ensure(6 * (size_t)NN * nc * 8);
+    c.d_mbar.ensure(4 * (size_t)NN * nc * 8);
+    c.d_xbar.ensure(3 * (size_t)np * nc * 8);
+    c.d_pbar.ensure((3 * (size_t)nc + 3) * 8);
+    c.d_abar.ensure(6 * (size_t)nc * 8);
+    c.d_costab.ensure(MAX_NP * 8);
+    for (int i = 0; i < MAX_NP; ++i)
+        c.costab_host[i] = i < c.np ? cos(M_PI * (double)i / (double)c.n) : 0.0;  // cheb2.jl:4
+    FMM_HIP(hipMemcpyAsync(c.d_costab.p, c.costab_host, sizeof c.costab_host, hipMemcpyHostToDevice,
+                           st));
+    transpose_lists(c, c.p2p_sorted, c.np2p, c.d_p2pT, c.d_p2pT_alt, c.d_p2pT_off, &c.p2pT_sorted);
+    transpose_lists(c, c.m2l_sorted, c.nm2l, c.d_m2lT, c.d_m2lT_alt, c.d_m2lT_off, &c.m2lT_sorted);
+
+    double *ob = c.d_ob.as<double>(), *gx = c.d_gx.as<double>(), *gp = c.d_gp.as<double>();
+    double *lbar = c.d_lbar.as<double>(), *mbar = c.d_mbar.as<double>();
+    double *xbar = c.d_xbar.as<double>();
+    hipLaunchKernelGGL(k_adj_prep, dim3(nbk(N, 256)), dim3(256), 0, st, N,
+                       c.d_parind.as<int32_t>(), Ebar, Bbar, amp, ob, gx, gp);
+    FMM_LAUNCH_CHECK();
+    const int64_t nx = 3 * np * nc;
+    hipLaunchKernelGGL(k_fill0, dim3(nbk(nx, 256)), dim3(256), 0, st, xbar, nx);
+    FMM_LAUNCH_CHECK();
+    const int32_t *leaves = c.d_leaves.as<int32_t>(), *lo = c.d_lo.as<int32_t>(),
+                  *cnt = c.d_cnt.as<int32_t>();
+    const double *sp8 = c.d_sp8.as<double>(), *ncoord = c.d_ncoord.as<double>();
+    double *loc = c.d_loc.as<double>(), *pbar = c.d_pbar.as<double>();
+
+    // 1. local cotangents (values only; they do not depend on the forward locals):
+    //    L2P^T at the leaves, then L2L^T bottom-up
+#define ADJ_L2PV(NP)                                                                                \
+    hipLaunchKernelGGL((k_adj_leaf<NP, 0, 2>), dim3(nleaf), dim3(AT), 0, st, leaves, lo, cnt, sp8,  \
+                       ncoord, loc, ob, lbar, gx, gp, xbar)
+    ADJ_NP_SWITCH(c.np, ADJ_L2PV);
+    FMM_LAUNCH_CHECK();
+    for (int l = S.ml; l >= 1; --l) {
+        const int64_t b = S.split_off[l - 1], e = S.split_off[l];
+        if (e == b) continue;
+        const int32_t *par = c.d_lvl_split.as<int32_t>() + b;
+#define ADJ_L2LV(NP)                                                                                \
+    hipLaunchKernelGGL((k_adj_transfer<NP, 0, 2>), dim3((unsigned)(e - b)), dim3(AT), 0, st, par, \
+                       ncoord, lbar, loc, lbar, xbar)
+        ADJ_NP_SWITCH(c.np, ADJ_L2LV);
+        FMM_LAUNCH_CHECK();
+    }
+    // 2. forward M2L fused with the M2L^T target side, then the forward L2L
+#define ADJ_M2LT(NP)                                                                               \
+    hipLaunchKernelGGL((k_adj_m2l_t<NP, true>), dim3((unsigned)nc), dim3(AShape<NP>::BLK), 0, st, \
+                       c.d_m2l_off.as<int32_t>(), c.m2l_sorted, c.sb, ncoord,                      \
+                       c.d_mp4.as<double>(), lbar, c.d_pavg.as<double>(), xbar, pbar, loc)
+    ADJ_NP_SWITCH(c.np, ADJ_M2LT);
+    FMM_LAUNCH_CHECK();
+    launch_l2l(c);
+    FMM_HIP(hipEventRecord(c.ev[8], st));
+    // 3. geometry of L2P^T and L2L^T (needs the final forward locals)
+#define ADJ_L2PG(NP)                                                                                \
+    hipLaunchKernelGGL((k_adj_leaf<NP, 0, 1>), dim3(nleaf), dim3(AT), 0, st, leaves, lo, cnt, sp8,  \
+                       ncoord, loc, ob, lbar, gx, gp, xbar)
+    ADJ_NP_SWITCH(c.np, ADJ_L2PG);
+    FMM_LAUNCH_CHECK();
+    for (int l = S.ml; l >= 1; --l) {
+        const int64_t b = S.split_off[l - 1], e = S.split_off[l];
+        if (e == b) continue;
+        const int32_t *par = c.d_lvl_split.as<int32_t>() + b;
+#define ADJ_L2LG(NP)                                                                                \
+    hipLaunchKernelGGL((k_adj_transfer<NP, 0, 1>), dim3((unsigned)(e - b)), dim3(AT), 0, st, par, \
+                       ncoord, lbar, loc, lbar, xbar)
+        ADJ_NP_SWITCH(c.np, ADJ_L2LG);
+        FMM_LAUNCH_CHECK();
+    }
+    // 4. M2L^T source side
+#define ADJ_M2LS(NP)                                                                              \
+    hipLaunchKernelGGL((k_adj_m2l_s<NP>), dim3((unsigned)nc), dim3(AShape<NP>::BLK), 0, st,      \
+                       c.d_m2lT_off.as<int32_t>(), c.m2lT_sorted, c.sb, ncoord,                   \
+                       c.d_mp4.as<double>(), lbar, c.d_pavg.as<double>(), mbar, xbar)
+    ADJ_NP_SWITCH(c.np, ADJ_M2LS);
+    FMM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_adj_pbar, dim3(1), dim3(256), 0, st, pbar, nc, pbar + 3 * nc);
+    FMM_LAUNCH_CHECK();
+    FMM_HIP(hipEventRecord(c.ev[9], st));
+    // 5. M2M^T, top-down: parents on level l-1 for l = 1 .. ml
+    for (int l = 1; l <= S.ml; ++l) {
+        const int64_t b = S.split_off[l - 1], e = S.split_off[l];
+        if (e == b) continue;
+        const int32_t *par = c.d_lvl_split.as<int32_t>() + b;
+#define ADJ_M2M(NP)                                                                                 \
+    hipLaunchKernelGGL((k_adj_transfer<NP, 1, 3>), dim3((unsigned)(e - b)), dim3(AT), 0, st, par, \
+                       ncoord, c.d_mp4.as<double>(), mbar, mbar, xbar)
+        ADJ_NP_SWITCH(c.np, ADJ_M2M);
+        FMM_LAUNCH_CHECK();
+    }
+    // 6. P2M^T
+#define ADJ_P2M(NP)                                                                                 \
+    hipLaunchKernelGGL((k_adj_leaf<NP, 1, 3>), dim3(nleaf), dim3(AT), 0, st, leaves, lo, cnt, sp8,  \
+                       ncoord, mbar, ob, (double *)nullptr, gx, gp, xbar)
+    ADJ_NP_SWITCH(c.np, ADJ_P2M);
+    FMM_LAUNCH_CHECK();
+    // 7. P2P^T
+    int bt = ((S.maxleaf + 63) / 64) * 64;
+    bt = std::max(64, std::min(bt, PT));
+    hipLaunchKernelGGL(k_adj_p2p_t, dim3(nleaf), dim3(bt), 0, st, leaves,
+                       c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, lo, cnt, sp8, ob, gx);
+    FMM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_adj_p2p_s, dim3(nleaf), dim3(bt), 0, st, leaves,
+                       c.d_p2pT_off.as<int32_t>(), c.p2pT_sorted, c.sb, lo, cnt, sp8, ob, gx, gp);
+    FMM_LAUNCH_CHECK();
+    // bounding boxes -> extreme particles, mean momentum, output order
+    hipLaunchKernelGGL(k_adj_bbox, dim3(nbk(nc, 128)), dim3(128), 0, st, nc, c.np,
+                       c.d_flags.as<uint8_t>(), xbar, c.d_costab.as<double>(),
+                       c.d_abar.as<double>());
+    FMM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_adj_finish, dim3(nbk(N, 256)), dim3(256), 0, st, N,
+                       c.d_parind.as<int32_t>(), c.d_leafof.as<int32_t>(), c.d_ext.as<int32_t>(),
+                       c.d_abar.as<double>(), pbar + 3 * nc, gx, gp, gpos, gmom);
+    FMM_LAUNCH_CHECK();
+}
+
+}  // namespace fmm4rb

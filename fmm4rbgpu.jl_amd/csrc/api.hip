@@ -159,6 +159,7 @@ static void prepare(Ctx &c, int64_t N, int64_t N0, int n, int precision) {
     c.d_fbuf.ensure(n3);
     c.d_bsum.ensure(4 * (3 * (size_t)((N + 2047) / 2048) + 3));
     c.d_axis.ensure(nc);
+    c.d_ext.ensure(6 * nc * 4);
     c.d_parind.ensure((size_t)N * 4);
     c.d_leafof.ensure((size_t)N * 4);
     c.d_bbox.ensure(6 * nc * 8);
@@ -300,13 +301,65 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
     }
 }
 
+// Reverse mode: forward recompute of the expansions (no P2P / L2P), then the adjoint.
+static void run_vjp(Ctx &c, int64_t N, const double *pos, const double *mom, double charge,
+                    double lambda, int n, int64_t N0, double eta, int precision,
+                    const double *Ebar, const double *Bbar, double *gpos, double *gmom,
+                    fmm4rb_stats *stats) {
+    validate(N, n, N0, eta, precision);
+    if (precision != FMM4RB_FP64)
+        fail(FMM4RB_EINVAL, "the adjoint is available in FMM4RB_FP64 only");
+    if (!std::isfinite(charge) || !std::isfinite(lambda) || lambda == 0.0)
+        fail(FMM4RB_EINVAL, "charge must be finite and lambda finite and non-zero");
+    prepare(c, N, N0, n, precision);
+    hipStream_t st = c.stream;
+    const double amp = (AMP_RE * charge) / lambda;
+    FMM_HIP(hipEventRecord(c.ev[0], st));
+    stage_pavg(c, pos, mom);
+    FMM_HIP(hipEventRecord(c.ev[1], st));
+    build_tree_device(c, pos, mom, N, N0);
+    launch_node_geo(c);
+    FMM_HIP(hipEventRecord(c.ev[2], st));
+    launch_upward(c);
+    FMM_HIP(hipEventRecord(c.ev[3], st));
+    build_lists_device(c, eta);
+    FMM_HIP(hipEventRecord(c.ev[4], st));
+    FMM_HIP(hipEventRecord(c.ev[5], st));
+    run_adjoint(c, Ebar, Bbar, amp, gpos, gmom);
+    FMM_HIP(hipEventRecord(c.ev[6], st));
+    FMM_HIP(hipStreamSynchronize(st));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->N = N;
+        stats->N0 = N0;
+        stats->ncluster = c.shape.nc;
+        stats->nleaf = (int64_t)c.shape.leaves.size();
+        stats->n = n;
+        stats->precision = precision;
+        stats->maxlevel = c.shape.ml;
+        stats->traversal_iterations = c.trav_iters;
+        stats->np2p = c.np2p;
+        stats->nm2l = c.nm2l;
+        for (int d = 0; d < 3; ++d) stats->p_avg[d] = c.pavg_host[d];
+        stats->g_avg = c.pavg_host[3];
+        FMM_HIP(hipEventElapsedTime(&stats->ms_total, c.ev[0], c.ev[6]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_pavg, c.ev[0], c.ev[1]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_tree, c.ev[1], c.ev[2]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_upward, c.ev[2], c.ev[3]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_lists, c.ev[3], c.ev[4]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_m2l, c.ev[5], c.ev[8]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_p2p, c.ev[8], c.ev[9]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_downward, c.ev[9], c.ev[6]));
+    }
+}
+
 }  // namespace fmm4rb
 
 using namespace fmm4rb;
 
 struct fmm4rb_ctx {
     Ctx c;
-    DevBuf h_pos, h_mom, h_E, h_B, h_tg;
+    DevBuf h_pos, h_mom, h_E, h_B, h_tg, h_gpos, h_gmom, h_ob;
 };
 
 #define FMM_API_BEGIN try {
@@ -365,10 +418,14 @@ int fmm4rb_destroy(fmm4rb_ctx *ctx) {
                       &c.d_leftbase, &c.d_part, &c.d_pavg, &c.d_bad, &c.d_keys, &c.d_keys_alt,
                       &c.d_vals, &c.d_vals_alt, &c.d_rs_scratch, &c.d_seg, &c.d_seg_alt,
                       &c.d_flag, &c.d_fbuf, &c.d_bsum, &c.d_axis, &c.d_parind, &c.d_leafof,
-                      &c.d_bbox, &c.d_geo, &c.d_ncoord, &c.d_costab, &c.d_sp8, &c.d_sp8f,
+                      &c.d_bbox, &c.d_geo, &c.d_ncoord, &c.d_costab, &c.d_ext, &c.d_sp8, &c.d_sp8f,
+                      &c.d_ob, &c.d_gx, &c.d_gp, &c.d_lbar, &c.d_mbar, &c.d_xbar, &c.d_pbar,
+                      &c.d_abar, &c.d_p2pT, &c.d_p2pT_alt, &c.d_m2lT, &c.d_m2lT_alt,
+                      &c.d_p2pT_off, &c.d_m2lT_off,
                       &c.d_front0, &c.d_front1, &c.d_tcnt, &c.d_p2p, &c.d_p2p_alt, &c.d_m2l,
                       &c.d_m2l_alt, &c.d_p2p_off, &c.d_m2l_off, &c.d_mp4, &c.d_mp4f, &c.d_loc,
-                      &c.d_near, &ctx->h_pos, &ctx->h_mom, &ctx->h_E, &ctx->h_B, &ctx->h_tg};
+                      &c.d_near, &ctx->h_pos, &ctx->h_mom, &ctx->h_E, &ctx->h_B, &ctx->h_tg,
+                      &ctx->h_gpos, &ctx->h_gmom, &ctx->h_ob};
     for (DevBuf *b : bufs) b->release();
     for (auto &e : c.ev)
         if (e) (void)hipEventDestroy(e);
@@ -420,6 +477,78 @@ int fmm4rb_field(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mo
               eta, precision, ctx->h_E.as<double>(), ctx->h_B.as<double>(), stats);
     FMM_HIP(hipMemcpyAsync(E, ctx->h_E.p, bytes, hipMemcpyDeviceToHost, st));
     FMM_HIP(hipMemcpyAsync(B, ctx->h_B.p, bytes, hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipStreamSynchronize(st));
+    FMM_API_END
+}
+
+int fmm4rb_field_vjp_device(fmm4rb_ctx *ctx, int64_t N, const double *d_pos, const double *d_mom,
+                            double charge, double lambda, int n, int64_t N0, double eta,
+                            int precision, const double *d_Ebar, const double *d_Bbar,
+                            double *d_gpos, double *d_gmom, fmm4rb_stats *stats) {
+    FMM_API_BEGIN
+    FMM_NEED(ctx && d_pos && d_mom && d_Ebar && d_Bbar && d_gpos && d_gmom, "NULL argument");
+    set_device(ctx);
+    run_vjp(ctx->c, N, d_pos, d_mom, charge, lambda, n, N0, eta, precision, d_Ebar, d_Bbar,
+            d_gpos, d_gmom, stats);
+    FMM_API_END
+}
+
+int fmm4rb_field_vjp(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
+                     double charge, double lambda, int n, int64_t N0, double eta, int precision,
+                     const double *Ebar, const double *Bbar, double *gpos, double *gmom,
+                     fmm4rb_stats *stats) {
+    FMM_API_BEGIN
+    FMM_NEED(ctx && pos && mom && Ebar && Bbar && gpos && gmom, "NULL argument");
+    validate(N, n, N0, eta, precision);
+    set_device(ctx);
+    hipStream_t st = ctx->c.stream;
+    const size_t bytes = 3 * (size_t)N * sizeof(double);
+    DevBuf *b[] = {&ctx->h_pos, &ctx->h_mom, &ctx->h_E, &ctx->h_B, &ctx->h_gpos, &ctx->h_gmom};
+    for (DevBuf *x : b) x->ensure(bytes);
+    FMM_HIP(hipMemcpyAsync(ctx->h_pos.p, pos, bytes, hipMemcpyHostToDevice, st));
+    FMM_HIP(hipMemcpyAsync(ctx->h_mom.p, mom, bytes, hipMemcpyHostToDevice, st));
+    FMM_HIP(hipMemcpyAsync(ctx->h_E.p, Ebar, bytes, hipMemcpyHostToDevice, st));
+    FMM_HIP(hipMemcpyAsync(ctx->h_B.p, Bbar, bytes, hipMemcpyHostToDevice, st));
+    run_vjp(ctx->c, N, ctx->h_pos.as<double>(), ctx->h_mom.as<double>(), charge, lambda, n, N0,
+            eta, precision, ctx->h_E.as<double>(), ctx->h_B.as<double>(),
+            ctx->h_gpos.as<double>(), ctx->h_gmom.as<double>(), stats);
+    FMM_HIP(hipMemcpyAsync(gpos, ctx->h_gpos.p, bytes, hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(gmom, ctx->h_gmom.p, bytes, hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipStreamSynchronize(st));
+    FMM_API_END
+}
+
+int fmm4rb_bruteforce_vjp(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
+                          double charge, double lambda, const double *Ebar, const double *Bbar,
+                          double *gpos, double *gmom) {
+    FMM_API_BEGIN
+    FMM_NEED(ctx && pos && mom && Ebar && Bbar && gpos && gmom, "NULL argument");
+    FMM_NEED(N >= 1 && N < 0x7fffffffLL, "N out of range");
+    FMM_NEED(std::isfinite(charge) && std::isfinite(lambda) && lambda != 0.0,
+             "charge must be finite and lambda finite and non-zero");
+    set_device(ctx);
+    hipStream_t st = ctx->c.stream;
+    const double amp = (AMP_RE * charge) / lambda;
+    const size_t bytes = 3 * (size_t)N * sizeof(double);
+    std::vector<double> ob(6 * (size_t)N);
+    for (int64_t i = 0; i < N; ++i)
+        for (int d = 0; d < 3; ++d) {
+            ob[6 * i + d] = amp * Ebar[3 * i + d];
+            ob[6 * i + 3 + d] = amp * Bbar[3 * i + d];
+        }
+    ctx->h_pos.ensure(bytes);
+    ctx->h_mom.ensure(bytes);
+    ctx->h_ob.ensure(2 * bytes);
+    ctx->h_gpos.ensure(bytes);
+    ctx->h_gmom.ensure(bytes);
+    FMM_HIP(hipMemcpyAsync(ctx->h_pos.p, pos, bytes, hipMemcpyHostToDevice, st));
+    FMM_HIP(hipMemcpyAsync(ctx->h_mom.p, mom, bytes, hipMemcpyHostToDevice, st));
+    FMM_HIP(hipMemcpyAsync(ctx->h_ob.p, ob.data(), 2 * bytes, hipMemcpyHostToDevice, st));
+    launch_bruteforce_vjp(ctx->h_pos.as<double>(), ctx->h_mom.as<double>(), N,
+                          ctx->h_ob.as<double>(), ctx->h_gpos.as<double>(),
+                          ctx->h_gmom.as<double>(), st);
+    FMM_HIP(hipMemcpyAsync(gpos, ctx->h_gpos.p, bytes, hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(gmom, ctx->h_gmom.p, bytes, hipMemcpyDeviceToHost, st));
     FMM_HIP(hipStreamSynchronize(st));
     FMM_API_END
 }

@@ -65,6 +65,131 @@ __global__ __launch_bounds__(BF_T) void k_bruteforce(const double *__restrict__ 
         }
 }
 
+// Reverse mode of BruteForce: for every ordered pair (i, j), u = g_j e_i + b_i x p_j,
+// G = d^-3/2 u - 3 d^-5/2 (R.u)(R + (p_j.R) p_j); gpos_i += G, gpos_j -= G,
+// gmom_j += (e_i.K) p_j/g_j + K x b_i - 3 d^-5/2 (p_j.R)(R.u) R.  R = 0 pairs add nothing.
+// ob holds amp * (Ebar, Bbar) per particle (original order).  Target side:
+__global__ __launch_bounds__(BF_T) void k_bf_vjp_t(const double *__restrict__ pos,
+                                                   const double *__restrict__ mom, int64_t N,
+                                                   const double *__restrict__ ob,
+                                                   double *__restrict__ gpos) {
+    __shared__ double tile[BF_T][8];
+    const int64_t i = (int64_t)blockIdx.x * BF_T + threadIdx.x;
+    const bool act = i < N;
+    const int64_t ii = act ? i : 0;
+    const double x0 = pos[3 * ii], x1 = pos[3 * ii + 1], x2 = pos[3 * ii + 2];
+    const double e0 = ob[6 * ii], e1 = ob[6 * ii + 1], e2 = ob[6 * ii + 2];
+    const double b0 = ob[6 * ii + 3], b1 = ob[6 * ii + 4], b2 = ob[6 * ii + 5];
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    for (int64_t j0 = 0; j0 < N; j0 += BF_T) {
+        __syncthreads();
+        const int64_t j = j0 + threadIdx.x;
+        if (j < N) {
+            const double m0 = mom[3 * j], m1 = mom[3 * j + 1], m2 = mom[3 * j + 2];
+            tile[threadIdx.x][0] = pos[3 * j];
+            tile[threadIdx.x][1] = pos[3 * j + 1];
+            tile[threadIdx.x][2] = pos[3 * j + 2];
+            tile[threadIdx.x][3] = sqrt(1.0 + ((m0 * m0 + m1 * m1) + m2 * m2));
+            tile[threadIdx.x][4] = m0;
+            tile[threadIdx.x][5] = m1;
+            tile[threadIdx.x][6] = m2;
+        }
+        __syncthreads();
+        const int m = (int)min<int64_t>(BF_T, N - j0);
+        if (act)
+            for (int jj = 0; jj < m; ++jj) {
+                const double *t = tile[jj];
+                const double R0 = x0 - t[0], R1 = x1 - t[1], R2 = x2 - t[2];
+                if (R0 == 0.0 && R1 == 0.0 && R2 == 0.0) continue;
+                const double px = t[4], py = t[5], pz = t[6], g = t[3];
+                const double pr = (px * R0 + py * R1) + pz * R2;
+                const double d = ((R0 * R0 + R1 * R1) + R2 * R2) + pr * pr + EPS64;
+                const double y = 1.0 / sqrt(d);
+                const double y3 = y * y * y, y5 = y3 * y * y;
+                const double u0 = g * e0 + (b1 * pz - b2 * py);
+                const double u1 = g * e1 + (b2 * px - b0 * pz);
+                const double u2 = g * e2 + (b0 * py - b1 * px);
+                const double Ru = (R0 * u0 + R1 * u1) + R2 * u2;
+                const double c3 = 3.0 * y5 * Ru;
+                g0 += y3 * u0 - c3 * (R0 + pr * px);
+                g1 += y3 * u1 - c3 * (R1 + pr * py);
+                g2 += y3 * u2 - c3 * (R2 + pr * pz);
+            }
+    }
+    if (act) {
+        gpos[3 * i] = g0;
+        gpos[3 * i + 1] = g1;
+        gpos[3 * i + 2] = g2;
+    }
+}
+
+// Source side (adds to gpos written by the target side).
+__global__ __launch_bounds__(BF_T) void k_bf_vjp_s(const double *__restrict__ pos,
+                                                   const double *__restrict__ mom, int64_t N,
+                                                   const double *__restrict__ ob,
+                                                   double *__restrict__ gpos,
+                                                   double *__restrict__ gmom) {
+    __shared__ double tile[BF_T][9];
+    const int64_t j = (int64_t)blockIdx.x * BF_T + threadIdx.x;
+    const bool act = j < N;
+    const int64_t jj = act ? j : 0;
+    const double x0 = pos[3 * jj], x1 = pos[3 * jj + 1], x2 = pos[3 * jj + 2];
+    const double px = mom[3 * jj], py = mom[3 * jj + 1], pz = mom[3 * jj + 2];
+    const double g = sqrt(1.0 + ((px * px + py * py) + pz * pz));
+    double gx0 = 0.0, gx1 = 0.0, gx2 = 0.0, gp0 = 0.0, gp1 = 0.0, gp2 = 0.0;
+    for (int64_t i0 = 0; i0 < N; i0 += BF_T) {
+        __syncthreads();
+        const int64_t i = i0 + threadIdx.x;
+        if (i < N) {
+            for (int d = 0; d < 3; ++d) tile[threadIdx.x][d] = pos[3 * i + d];
+            for (int d = 0; d < 6; ++d) tile[threadIdx.x][3 + d] = ob[6 * i + d];
+        }
+        __syncthreads();
+        const int m = (int)min<int64_t>(BF_T, N - i0);
+        if (act)
+            for (int k = 0; k < m; ++k) {
+                const double *t = tile[k];
+                const double R0 = t[0] - x0, R1 = t[1] - x1, R2 = t[2] - x2;
+                if (R0 == 0.0 && R1 == 0.0 && R2 == 0.0) continue;
+                const double e0 = t[3], e1 = t[4], e2 = t[5], b0 = t[6], b1 = t[7], b2 = t[8];
+                const double pr = (px * R0 + py * R1) + pz * R2;
+                const double d = ((R0 * R0 + R1 * R1) + R2 * R2) + pr * pr + EPS64;
+                const double y = 1.0 / sqrt(d);
+                const double y3 = y * y * y, y5 = y3 * y * y;
+                const double u0 = g * e0 + (b1 * pz - b2 * py);
+                const double u1 = g * e1 + (b2 * px - b0 * pz);
+                const double u2 = g * e2 + (b0 * py - b1 * px);
+                const double Ru = (R0 * u0 + R1 * u1) + R2 * u2;
+                const double c3 = 3.0 * y5 * Ru;
+                gx0 -= y3 * u0 - c3 * (R0 + pr * px);
+                gx1 -= y3 * u1 - c3 * (R1 + pr * py);
+                gx2 -= y3 * u2 - c3 * (R2 + pr * pz);
+                const double eK = y3 * ((e0 * R0 + e1 * R1) + e2 * R2);
+                const double cg = eK / g, cp = -c3 * pr;
+                gp0 += cg * px + cp * R0 + y3 * (R1 * b2 - R2 * b1);
+                gp1 += cg * py + cp * R1 + y3 * (R2 * b0 - R0 * b2);
+                gp2 += cg * pz + cp * R2 + y3 * (R0 * b1 - R1 * b0);
+            }
+    }
+    if (act) {
+        gpos[3 * j] += gx0;
+        gpos[3 * j + 1] += gx1;
+        gpos[3 * j + 2] += gx2;
+        gmom[3 * j] = gp0;
+        gmom[3 * j + 1] = gp1;
+        gmom[3 * j + 2] = gp2;
+    }
+}
+
+void launch_bruteforce_vjp(const double *pos, const double *mom, int64_t N, const double *ob,
+                           double *gpos, double *gmom, hipStream_t st) {
+    const unsigned g = (unsigned)((N + BF_T - 1) / BF_T);
+    hipLaunchKernelGGL(k_bf_vjp_t, dim3(g), dim3(BF_T), 0, st, pos, mom, N, ob, gpos);
+    FMM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bf_vjp_s, dim3(g), dim3(BF_T), 0, st, pos, mom, N, ob, gpos, gmom);
+    FMM_LAUNCH_CHECK();
+}
+
 void launch_bruteforce(const double *pos, const double *mom, int64_t N, const int64_t *targets,
                        int64_t nt, double amp, double *E, double *B, hipStream_t st) {
     hipLaunchKernelGGL(k_bruteforce, dim3((unsigned)((nt + BF_T - 1) / BF_T)), dim3(BF_T), 0, st,

@@ -51,6 +51,14 @@ constexpr double EPS64 = 2.220446049250313e-16;  // Julia eps()
 constexpr float EPS32K = 2.220446049250313e-16f;  // the same softening in FP32
 constexpr double AMP_RE = 2.8179403699772166e-15;
 constexpr int MAX_NP = 11;      // n <= 10
+
+// 1/sqrt(d) in FP64: the hardware estimate v_rsq_f64 (~5e-8 relative) refined by
+// one Newton step, y (1 + (1 - d y^2)/2): ~4e-15 relative (measured on gfx950).
+__device__ inline double rsqrt_nr(double d) {
+    const double y = __builtin_amdgcn_rsq(d);
+    const double r = fma(-d * y, y, 1.0);
+    return fma(y * 0.5, r, y);
+}
 constexpr int MAX_N0 = 1024;
 
 // Tree shape: a closed-form function of (N, N0) (clusters.jl:21-49 splits a
@@ -103,6 +111,14 @@ void launch_mp_to_f32(Ctx &c);
 void launch_m2l(Ctx &c, const int32_t *off, const uint64_t *keys, int sb, int64_t ngroups_grid);
 void launch_p2p(Ctx &c, const int32_t *off, const uint64_t *keys, int sb);
 
+// ---- adjoint (adjoint.hip, traverse.hip)
+void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, double *gpos,
+                 double *gmom);
+void transpose_lists(Ctx &c, const uint64_t *sorted, int64_t n, DevBuf &kT, DevBuf &kT_alt,
+                     DevBuf &offT, uint64_t **sortedT);
+void launch_bruteforce_vjp(const double *pos, const double *mom, int64_t N, const double *ob,
+                           double *gpos, double *gmom, hipStream_t st);
+
 // ---- brute force (bruteforce.hip)
 void launch_bruteforce(const double *pos, const double *mom, int64_t N, const int64_t *targets,
                        int64_t nt, double amp, double *E, double *B, hipStream_t st);
@@ -126,7 +142,7 @@ struct Ctx {
     DevBuf d_keys, d_keys_alt, d_vals, d_vals_alt, d_rs_scratch;
     DevBuf d_seg, d_seg_alt, d_flag, d_fbuf, d_bsum, d_axis;
     // tree outputs
-    DevBuf d_parind, d_leafof, d_bbox, d_geo, d_ncoord, d_costab;
+    DevBuf d_parind, d_leafof, d_bbox, d_geo, d_ncoord, d_costab, d_ext;
     DevBuf d_sp8, d_sp8f;
     // lists
     DevBuf d_front0, d_front1, d_tcnt, d_p2p, d_p2p_alt, d_m2l, d_m2l_alt, d_p2p_off, d_m2l_off;
@@ -137,10 +153,15 @@ struct Ctx {
     uint64_t *p2p_sorted = nullptr, *m2l_sorted = nullptr;
     // expansions and fields
     DevBuf d_mp4, d_mp4f, d_loc, d_near;
+    // adjoint (field_vjp)
+    DevBuf d_ob, d_gx, d_gp, d_lbar, d_mbar, d_xbar, d_pbar, d_abar;
+    DevBuf d_p2pT, d_p2pT_alt, d_m2lT, d_m2lT_alt, d_p2pT_off, d_m2lT_off;
+    uint64_t *p2pT_sorted = nullptr, *m2lT_sorted = nullptr;
     // debug snapshots
     std::vector<double> snap_loc_m2l;
     // host-side p_avg of the last call
     double pavg_host[4] = {0, 0, 0, 1};
+    double costab_host[MAX_NP] = {};
 };
 
 }  // namespace fmm4rb

@@ -32,7 +32,7 @@ struct M2LShape {
 template <typename T>
 __device__ inline T rsq(T d);
 template <>
-__device__ inline double rsq<double>(double d) { return rsqrt(d); }
+__device__ inline double rsq<double>(double d) { return rsqrt_nr(d); }
 template <>
 __device__ inline float rsq<float>(float d) { return __builtin_amdgcn_rsqf(d); }
 

@@ -72,7 +72,7 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
                         const T d = fma(pr, pr, rr) + (T)EPS64;
                         T y;
                         if (MIXED) y = __builtin_amdgcn_rsqf((float)d);
-                        else y = (T)rsqrt((double)d);
+                        else y = (T)rsqrt_nr((double)d);
                         const T y3 = (y * y) * y;
                         const T gw = q.x * y3;
                         e0 = fma(gw, R0, e0);

@@ -129,6 +129,28 @@ void sort_pairs_csr(Ctx &c, uint64_t *keys, uint64_t *keys_alt, int64_t n, int32
     FMM_LAUNCH_CHECK();
 }
 
+__global__ void k_transpose_keys(const uint64_t *__restrict__ in, int64_t n, int sb,
+                                 uint64_t *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = in[i], mask = (1ull << sb) - 1ull;
+    out[i] = ((k & mask) << sb) | (k >> sb);
+}
+
+// Pairs regrouped by source: keys (s << sb) | t sorted, CSR offsets per source.
+void transpose_lists(Ctx &c, const uint64_t *sorted, int64_t n, DevBuf &kT, DevBuf &kT_alt,
+                     DevBuf &offT, uint64_t **sortedT) {
+    kT.ensure(8 * (size_t)std::max<int64_t>(n, 1));
+    kT_alt.ensure(8 * (size_t)std::max<int64_t>(n, 1));
+    offT.ensure(4 * (size_t)(c.shape.nc + 1));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_transpose_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           c.stream, sorted, n, c.sb, kT.as<uint64_t>());
+        FMM_LAUNCH_CHECK();
+    }
+    sort_pairs_csr(c, kT.as<uint64_t>(), kT_alt.as<uint64_t>(), n, offT.as<int32_t>(), sortedT);
+}
+
 void build_lists_device(Ctx &c, double eta) {
     const Shape &S = c.shape;
     hipStream_t st = c.stream;

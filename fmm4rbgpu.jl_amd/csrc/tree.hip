@@ -127,17 +127,20 @@ __global__ void k_level_nodes(int nnodes, const int32_t *__restrict__ nodes,
                               const uint8_t *__restrict__ flags, const uint32_t *__restrict__ L,
                               int64_t N, const double *__restrict__ pos,
                               const double *__restrict__ pavg4, double *__restrict__ bbox,
-                              int8_t *__restrict__ axis) {
+                              int8_t *__restrict__ axis, int32_t *__restrict__ ext) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nnodes) return;
     int32_t k = nodes[t];
     int64_t a = lo[k], b = (int64_t)lo[k] + cnt[k] - 1;
     double mn[3], mx[3];
     for (int d = 0; d < 3; ++d) {
-        mn[d] = pos[3 * (int64_t)L[d * N + a] + d] + 0.0;
-        mx[d] = pos[3 * (int64_t)L[d * N + b] + d] + 0.0;
+        const uint32_t pa = L[d * N + a], pb = L[d * N + b];  // argmin / argmax particle
+        mn[d] = pos[3 * (int64_t)pa + d] + 0.0;
+        mx[d] = pos[3 * (int64_t)pb + d] + 0.0;
         bbox[6 * (int64_t)k + d] = mn[d];
         bbox[6 * (int64_t)k + 3 + d] = mx[d];
+        ext[6 * (int64_t)k + d] = (int32_t)pa;
+        ext[6 * (int64_t)k + 3 + d] = (int32_t)pb;
     }
     if ((flags[k] & 3) == 1) {
         double s[3] = {1.0, 1.0, pavg4[3]};
@@ -305,7 +308,7 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
             hipLaunchKernelGGL(k_level_nodes, dim3(nb(nnodes, 128)), dim3(128), 0, st, nnodes,
                                c.d_lvl_nodes.as<int32_t>() + S.lvl_off[l], lo, cnt, flags, L, N,
                                pos, c.d_pavg.as<double>(), c.d_bbox.as<double>(),
-                               c.d_axis.as<int8_t>());
+                               c.d_axis.as<int8_t>(), c.d_ext.as<int32_t>());
             FMM_LAUNCH_CHECK();
         }
         int nsplit = (int)(S.split_off[l + 1] - S.split_off[l]);

@@ -31,7 +31,7 @@ EXPORTS = (
     "fmm4rb_version", "fmm4rb_last_error", "fmm4rb_create", "fmm4rb_destroy",
     "fmm4rb_set_stream", "fmm4rb_field", "fmm4rb_field_device", "fmm4rb_bruteforce",
     "fmm4rb_tree", "fmm4rb_lists", "fmm4rb_op_m2l", "fmm4rb_op_p2p", "fmm4rb_set_debug",
-    "fmm4rb_debug_fetch",
+    "fmm4rb_debug_fetch", "fmm4rb_field_vjp", "fmm4rb_field_vjp_device", "fmm4rb_bruteforce_vjp",
 )
 
 
@@ -97,6 +97,13 @@ def lib():
         L.fmm4rb_op_p2p.argtypes = [_vp, C.c_int, _i64, _dp, _dp, _ip, _i64, _ip, _ip, _vp, _i64,
                                     _vp, _dp, _dp]
         L.fmm4rb_debug_fetch.argtypes = [_vp, C.c_int, _dp, _i64]
+        L.fmm4rb_field_vjp.argtypes = [_vp, _i64, _dp, _dp, C.c_double, C.c_double, C.c_int, _i64,
+                                       C.c_double, C.c_int, _dp, _dp, _dp, _dp, C.POINTER(Stats)]
+        L.fmm4rb_field_vjp_device.argtypes = [_vp, _i64, _vp, _vp, C.c_double, C.c_double, C.c_int,
+                                              _i64, C.c_double, C.c_int, _vp, _vp, _vp, _vp,
+                                              C.POINTER(Stats)]
+        L.fmm4rb_bruteforce_vjp.argtypes = [_vp, _i64, _dp, _dp, C.c_double, C.c_double, _dp, _dp,
+                                            _dp, _dp]
         for name in EXPORTS:
             getattr(L, name)
         _lib = L
@@ -174,6 +181,34 @@ class Context:
         _check(lib().fmm4rb_field_device(self._h, N, pos_ptr, mom_ptr, charge, lam, n, N0, eta,
                                          _PREC[precision], E_ptr, B_ptr, C.byref(st)))
         return st
+
+    def field_vjp(self, pos, mom, Ebar, Bbar, charge=-1.0, lam=1.0, n=4, N0=125, eta=0.5,
+                  precision="fp64"):
+        """Cotangents of the fields -> (gpos, gmom, stats) (reverse mode, fixed tree)."""
+        pos, mom = _f64(pos), _f64(mom)
+        N = pos.shape[0]
+        gx, gp = np.empty((N, 3)), np.empty((N, 3))
+        st = Stats()
+        _check(lib().fmm4rb_field_vjp(self._h, N, pos, mom, charge, lam, n, N0, eta,
+                                      _PREC[precision], _f64(Ebar), _f64(Bbar), gx, gp,
+                                      C.byref(st)))
+        return gx, gp, st
+
+    def field_vjp_device(self, N, pos_ptr, mom_ptr, Ebar_ptr, Bbar_ptr, gpos_ptr, gmom_ptr,
+                         charge=-1.0, lam=1.0, n=4, N0=125, eta=0.5, precision="fp64"):
+        st = Stats()
+        _check(lib().fmm4rb_field_vjp_device(self._h, N, pos_ptr, mom_ptr, charge, lam, n, N0,
+                                             eta, _PREC[precision], Ebar_ptr, Bbar_ptr,
+                                             gpos_ptr, gmom_ptr, C.byref(st)))
+        return st
+
+    def bruteforce_vjp(self, pos, mom, Ebar, Bbar, charge=-1.0, lam=1.0):
+        pos, mom = _f64(pos), _f64(mom)
+        N = pos.shape[0]
+        gx, gp = np.empty((N, 3)), np.empty((N, 3))
+        _check(lib().fmm4rb_bruteforce_vjp(self._h, N, pos, mom, charge, lam, _f64(Ebar),
+                                           _f64(Bbar), gx, gp))
+        return gx, gp
 
     def bruteforce(self, pos, mom, charge=-1.0, lam=1.0, targets=None):
         pos, mom = _f64(pos), _f64(mom)

@@ -76,6 +76,32 @@ int fmm4rb_field_device(fmm4rb_ctx *ctx, int64_t N, const double *d_pos, const d
                         int precision, double *d_E, double *d_B, fmm4rb_stats *stats);
 
 /*
+ * Reverse mode (no reference counterpart): the vector-Jacobian product of the
+ * field map at fixed tree and lists.  Given cotangents Ebar, Bbar (3N each,
+ * of the fields as fmm4rb_field returns them), writes gpos = sum Ebar.dE/dpos
+ * + Bbar.dB/dpos and gmom likewise (3N each), including the dependence of the
+ * interpolation nodes on the cluster bounding boxes and of the M2L kernel on
+ * the mean momentum.  The tree and lists are rebuilt, not stored.  For a loss
+ * L = sum_i w_i (|E_i|^2 + |B_i|^2), pass Ebar = 2 w E, Bbar = 2 w B.
+ * FMM4RB_FP64 only.  stats: ms_m2l = transposed lists, local cotangents and the
+ * forward M2L fused with the M2L^T target side plus the forward L2L; ms_p2p =
+ * L2P^T/L2L^T geometry and the M2L^T source side; ms_downward = M2M^T, P2M^T,
+ * P2P^T and the gradient assembly.
+ */
+int fmm4rb_field_vjp(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
+                     double charge, double lambda, int n, int64_t N0, double eta, int precision,
+                     const double *Ebar, const double *Bbar, double *gpos, double *gmom,
+                     fmm4rb_stats *stats);
+int fmm4rb_field_vjp_device(fmm4rb_ctx *ctx, int64_t N, const double *d_pos, const double *d_mom,
+                            double charge, double lambda, int n, int64_t N0, double eta,
+                            int precision, const double *d_Ebar, const double *d_Bbar,
+                            double *d_gpos, double *d_gmom, fmm4rb_stats *stats);
+/* Reverse mode of BruteForce (all N targets), FP64, host pointers. */
+int fmm4rb_bruteforce_vjp(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
+                          double charge, double lambda, const double *Ebar, const double *Bbar,
+                          double *gpos, double *gmom);
+
+/*
  * BruteForce on the GPU (src/update_particles_field.jl:21-38) for ntargets
  * particle indices (all N if targets == NULL, ntargets ignored).  Host
  * pointers; E, B hold 3 doubles per target.  FP64.

@@ -63,3 +63,39 @@ def test_accuracy_and_precision_2e20(ctx, bunch20, n):
     print(f"n={n} fp64-vs-BF E {W.rel_l2(E[tg], Eb):.3e} B {W.rel_l2(B[tg], Bb):.3e}; "
           f"mixed-vs-fp64 E {W.rel_l2(Em, E):.3e} B {W.rel_l2(Bm, B):.3e}; "
           f"np2p {st.np2p} nm2l {st.nm2l} ms {st.ms_total:.1f}")
+
+
+def test_adjoint_fd_2e20(ctx, bunch20):
+    """Adjoint at the N=2^20 bunch: 64 random coordinates vs central differences
+    (h = 1e-6 of the coordinate scale), fixed-topology coordinates only."""
+    pos, mom = bunch20
+    n, N0, eta, lam = 4, 125, 0.5, 1e-3
+    w = np.random.default_rng(40).random(len(pos))
+    E, B, fst = ctx.field(pos, mom, -1.0, lam, n, N0, eta)
+    gx, gp, st = ctx.field_vjp(pos, mom, 2 * w[:, None] * E, 2 * w[:, None] * B, -1.0, lam, n,
+                               N0, eta)
+    fields = lambda x, p: ctx.field(x, p, -1.0, lam, n, N0, eta)[:2]
+    topo = lambda x, p: (ctx.tree(x, p, N0)["parindices"], *ctx.lists(x, p, N0, eta))
+    same = lambda a, b: all(np.array_equal(u, v) for u, v in zip(a, b))
+    base = topo(pos, mom)
+    hs = (1e-6 * pos.std(axis=0).max(), 1e-6 * mom.std(axis=0).max())
+    rng = np.random.default_rng(41)
+    errs, skipped = [], 0
+    for i, a, d in zip(rng.integers(0, len(pos), 64), rng.integers(0, 2, 64), rng.integers(0, 3, 64)):
+        xs = [pos.copy(), mom.copy()]
+        xs[a][i, d] += hs[a]
+        (Ep, Bp), tp = fields(*xs), topo(*xs)
+        xs[a][i, d] -= 2 * hs[a]
+        (Em, Bm), tm = fields(*xs), topo(*xs)
+        if not (same(base, tp) and same(base, tm)):
+            skipped += 1
+            continue
+        # sum w (F+ - F-).(F+ + F-): the large loss itself never cancels
+        dl = (w[:, None] * ((Ep - Em) * (Ep + Em) + (Bp - Bm) * (Bp + Bm))).sum()
+        fd = float(dl) / (2 * hs[a])
+        g = (gx, gp)[a][i, d]
+        errs.append(abs(g - fd) / (abs(fd) + 1e-3 * np.sqrt(np.mean((gx, gp)[a] ** 2))))
+    errs = np.array(errs)
+    print(f"2^20 adjoint: {len(errs)} coords, median {np.median(errs):.2e}, max {errs.max():.2e}, "
+          f"skipped {skipped}; forward {fst.ms_total:.1f} ms, vjp {st.ms_total:.1f} ms")
+    assert skipped <= 16 and np.median(errs) <= 1e-5 and errs.max() <= 1e-2
