@@ -4,13 +4,15 @@
 # (src/update_particles_field.jl:9-19) plus a precision, and a method of
 # `update_particles_field!` for it.  When FMM4RBGPU is loaded the method extends
 # FMM4RBGPU.update_particles_field!; otherwise the module defines a standalone
-# `Particles`-compatible entry point `update_fields!`.
+# `Particles`-compatible entry point `update_fields!`.  `fields` is the
+# non-mutating form; when ChainRulesCore is loaded it gets an `rrule` whose
+# pullback calls the hand-written adjoint (fmm4rb_field_vjp).
 #
 # Julia is not installed where this library is built and tested; the C ABI is
 # what the test suite exercises (tests/test_gpu_parity.py through ctypes).
 module FMM4RBHIP
 
-export FMMHIP, update_fields!
+export FMMHIP, update_fields!, fields, field_vjp
 
 const libfmm4rb = get(ENV, "FMM4RB_HIP_LIB",
                       joinpath(@__DIR__, "..", "lib", "libfmm4rb_hip.so"))
@@ -75,6 +77,51 @@ function update_fields!(positions::AbstractVector, momenta::AbstractVector,
     return nothing
 end
 
+"""
+    fields(positions, momenta, charge, alg::FMMHIP; lambda) -> (E, B)
+
+Non-mutating form of `update_fields!`: new vectors of the positions' element type.
+"""
+function fields(positions::AbstractVector, momenta::AbstractVector, charge::Real, alg::FMMHIP;
+                lambda, ctx::Context = default_context())
+    E = similar(positions)
+    B = similar(positions)
+    update_fields!(positions, momenta, E, B, charge, alg; lambda = lambda, ctx = ctx)
+    return E, B
+end
+
+"""
+    field_vjp(positions, momenta, Ebar, Bbar, charge, alg::FMMHIP; lambda) -> (gpos, gmom)
+
+Reverse mode of `fields` at fixed tree and interaction lists (rebuilt, not stored):
+gpos = sum Ebar.dE/dpos + Bbar.dB/dpos, gmom likewise.  Computed in FP64 for either
+`alg.precision` (the mixed-precision fields agree with FP64 to 1e-5).
+"""
+function field_vjp(positions::AbstractVector, momenta::AbstractVector, Ebar::AbstractVector,
+                   Bbar::AbstractVector, charge::Real, alg::FMMHIP; lambda,
+                   ctx::Context = default_context())
+    N = length(positions)
+    @assert length(momenta) == N && length(Ebar) == N && length(Bbar) == N
+    gpos = similar(positions)
+    gmom = similar(momenta)
+    pos = reinterpret(Float64, positions)
+    mom = reinterpret(Float64, momenta)
+    eb = reinterpret(Float64, Ebar)
+    bb = reinterpret(Float64, Bbar)
+    gx = reinterpret(Float64, gpos)
+    gp = reinterpret(Float64, gmom)
+    GC.@preserve pos mom eb bb gx gp begin
+        check(ccall((:fmm4rb_field_vjp, libfmm4rb), Cint,
+                    (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Cint, Int64,
+                     Float64, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                     Ptr{Cvoid}),
+                    ctx.ptr, N, pointer(pos), pointer(mom), Float64(charge), Float64(lambda),
+                    Cint(alg.n), Int64(alg.N0), Float64(alg.eta), FP64, pointer(eb), pointer(bb),
+                    pointer(gx), pointer(gp), C_NULL))
+    end
+    return gpos, gmom
+end
+
 # Extend the reference's generic function when the reference package is present.
 function __init__()
     if isdefined(Main, :FMM4RBGPU)
@@ -83,6 +130,22 @@ function __init__()
                                                      alg::FMMHIP; lambda)
             update_fields!(particles.positions, particles.momenta, particles.efields,
                            particles.bfields, particles.charge, alg; lambda = lambda)
+        end
+    end
+    # ChainRulesCore rule for `fields` (charge, alg and lambda are parameters).
+    if isdefined(Main, :ChainRulesCore)
+        CR = getfield(Main, :ChainRulesCore)
+        @eval function $(CR).rrule(::typeof(fields), positions, momenta, charge, alg::FMMHIP;
+                                   lambda)
+            E, B = fields(positions, momenta, charge, alg; lambda = lambda)
+            function fields_pullback(ybar)
+                Eb, Bb = $(CR).unthunk(ybar)
+                Eb = Eb isa $(CR).AbstractZero ? zero(E) : collect(Eb)
+                Bb = Bb isa $(CR).AbstractZero ? zero(B) : collect(Bb)
+                gpos, gmom = field_vjp(positions, momenta, Eb, Bb, charge, alg; lambda = lambda)
+                return ($(CR).NoTangent(), gpos, gmom, $(CR).NoTangent(), $(CR).NoTangent())
+            end
+            return (E, B), fields_pullback
         end
     end
 end
