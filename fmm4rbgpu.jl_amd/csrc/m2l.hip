@@ -18,6 +18,7 @@
 // (v_rsq_f32), and each source pair's FP32 sum is added into FP64
 // accumulators.  The reference's LDS reuse without a barrier
 // (gpu_interaction.jl:15-38) is not repeated: a barrier precedes every refill.
+
 #include "fmm4rb_internal.hpp"
 
 namespace fmm4rb {

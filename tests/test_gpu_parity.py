@@ -132,7 +132,7 @@ def test_fp64_fmm_cases(ctx, orc, name):
     assert W.rel_l2(E, Eo) <= FP64_TOL and W.rel_l2(B, Bo) <= FP64_TOL
 
 
-@pytest.mark.parametrize("n", [4, 6])
+@pytest.mark.parametrize("n", [2, 4, 6, 7])
 def test_mixed_close_to_fp64(ctx, n):
     pos, mom = W.bunch(1 << 16, seed=8)
     E, B, _ = ctx.field(pos, mom, -1.0, 1e-3, n, 125, 0.5, "fp64")
@@ -187,6 +187,27 @@ def test_op_m2l_matches_oracle(ctx, orc):
     pavg = np.array([0.01, -0.02, 50.0])
     le, lb = ctx.op_m2l(n, bb, gam, mm, pairs, pavg, "fp64")
     leo, lbo = orc.m2l(n, bb, pairs, pavg, gam, mm)
+    assert W.rel_l2(le, leo) <= FP64_TOL and W.rel_l2(lb, lbo) <= FP64_TOL
+    lem, lbm = ctx.op_m2l(n, bb, gam, mm, pairs, pavg, "mixed")
+    assert W.rel_l2(lem, leo) <= MIXED_TOL and W.rel_l2(lbm, lbo) <= MIXED_TOL
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 10])
+def test_op_m2l_mixed_every_order(ctx, orc, n):
+    """Mixed-precision M2L (matrix-core form for n <= 6, vector form above) on
+    admissible pairs: 8 boxes on a grid of spacing 5, sides <= 1.5, all ordered
+    pairs, a relativistic mean momentum; FP64 locals and the oracle agree."""
+    rng = np.random.default_rng(40 + n)
+    nc = 8
+    cen = np.array([[5.0 * (i % 2), 5.0 * ((i // 2) % 2), 5.0 * (i // 4)] for i in range(nc)])
+    half = rng.uniform(0.2, 0.75, (nc, 3))
+    bb = np.concatenate([cen - half, cen + half], axis=1) + 1e3
+    nn = (n + 1) ** 3
+    gam, mm = 1 + rng.random(nn * nc), rng.standard_normal((nn * nc, 3))
+    pairs = [(t, s) for t in range(nc) for s in range(nc) if t != s]
+    pavg = np.array([0.3, -0.2, 20.0])
+    leo, lbo = orc.m2l(n, bb, pairs, pavg, gam, mm)
+    le, lb = ctx.op_m2l(n, bb, gam, mm, pairs, pavg, "fp64")
     assert W.rel_l2(le, leo) <= FP64_TOL and W.rel_l2(lb, lbo) <= FP64_TOL
     lem, lbm = ctx.op_m2l(n, bb, gam, mm, pairs, pavg, "mixed")
     assert W.rel_l2(lem, leo) <= MIXED_TOL and W.rel_l2(lbm, lbo) <= MIXED_TOL
