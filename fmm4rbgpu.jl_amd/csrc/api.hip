@@ -155,8 +155,8 @@ static void prepare(Ctx &c, int64_t N, int64_t N0, int n, int precision) {
     c.d_rs_scratch.ensure(4 * radix_scratch_words(N, 3));
     c.d_seg.ensure((size_t)N * 4);
     c.d_seg_alt.ensure((size_t)N * 4);
-    c.d_flag.ensure((size_t)N);
-    c.d_fbuf.ensure(n3);
+    c.d_rk.ensure(2 * 3 * n3 * 4);
+    c.d_pivot.ensure((size_t)nc * 4);
     c.d_bsum.ensure(4 * (3 * (size_t)((N + 2047) / 2048) + 3));
     c.d_axis.ensure(nc);
     c.d_ext.ensure(6 * nc * 4);
@@ -417,7 +417,7 @@ int fmm4rb_destroy(fmm4rb_ctx *ctx) {
     DevBuf *bufs[] = {&c.d_lo, &c.d_cnt, &c.d_flags, &c.d_leaves, &c.d_lvl_nodes, &c.d_lvl_split,
                       &c.d_leftbase, &c.d_part, &c.d_pavg, &c.d_bad, &c.d_keys, &c.d_keys_alt,
                       &c.d_vals, &c.d_vals_alt, &c.d_rs_scratch, &c.d_seg, &c.d_seg_alt,
-                      &c.d_flag, &c.d_fbuf, &c.d_bsum, &c.d_axis, &c.d_parind, &c.d_leafof,
+                      &c.d_rk, &c.d_pivot, &c.d_bsum, &c.d_axis, &c.d_parind, &c.d_leafof,
                       &c.d_bbox, &c.d_geo, &c.d_ncoord, &c.d_costab, &c.d_ext, &c.d_sp8, &c.d_sp8f,
                       &c.d_ob, &c.d_gx, &c.d_gp, &c.d_lbar, &c.d_mbar, &c.d_xbar, &c.d_pbar,
                       &c.d_abar, &c.d_p2pT, &c.d_p2pT_alt, &c.d_m2lT, &c.d_m2lT_alt,

@@ -140,7 +140,7 @@ struct Ctx {
     // tree build scratch
     DevBuf d_part, d_pavg, d_bad;
     DevBuf d_keys, d_keys_alt, d_vals, d_vals_alt, d_rs_scratch;
-    DevBuf d_seg, d_seg_alt, d_flag, d_fbuf, d_bsum, d_axis;
+    DevBuf d_seg, d_seg_alt, d_rk, d_pivot, d_bsum, d_axis;
     // tree outputs
     DevBuf d_parind, d_leafof, d_bbox, d_geo, d_ncoord, d_costab, d_ext;
     DevBuf d_sp8, d_sp8f;

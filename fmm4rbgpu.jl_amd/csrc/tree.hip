@@ -8,12 +8,16 @@
 // argmax(stretch .* (bmax - bmin)), and a quickselect that sends the
 // ceil(c/2) smallest coordinates left.  Here every level is processed at once:
 //   1. three stable radix sorts give, per axis d, the particle ids ordered by
-//      (coordinate_d, id)  -> lists L_d;
-//   2. at each level the lists stay segment-wise sorted, so a cluster's bbox
-//      is read off the first/last entry of each list, and the median split
-//      along its axis a is "first ceil(c/2) entries of L_a";
-//   3. a per-particle left/right flag then drives a stable segmented
-//      partition (global prefix sum) of all three lists.
+//      (coordinate_d, id) -> perm_d, and every particle's rank r_d in that order;
+//   2. three lists, one per axis, hold the particles of each cluster as rank
+//      triples (r_x, r_y, r_z), list d ordered by r_d within every cluster;
+//   3. at each level a cluster's bbox comes from the first/last entry of each
+//      list, and its median split along axis a sends left exactly the
+//      particles with r_a below the pivot rank (list a's entry at ceil(c/2)),
+//      so every list entry decides its side from its own triple, with no
+//      gather by particle id;
+//   4. a stable segmented partition (wave ballots + a global prefix sum) moves
+//      the three lists to the next level.
 // Ties in coordinate are broken by particle id.  Each leaf's ids are finally
 // sorted ascending (the canonical within-leaf order).
 #include <cmath>
@@ -29,6 +33,7 @@ void scan_rows(uint32_t *rows, int nrows, int len, uint32_t *tot, hipStream_t st
 constexpr int PAVG_THREADS = 256;
 constexpr int PAVG_MAXBLK = 1024;
 constexpr int PT_THREADS = 256, PT_ITEMS = 8, PT_TILE = PT_THREADS * PT_ITEMS;
+constexpr int PT_WAVES = PT_THREADS / 64;
 
 // ----------------------------------------------------------------- p_avg
 // Deterministic fixed-order sum of the momenta and a finiteness check of all
@@ -120,21 +125,42 @@ __global__ void k_make_keys(const double *__restrict__ pos, int64_t N, uint64_t 
     seg[i] = 0;
 }
 
-// Bounding box (clusters.jl:51-70) and split axis (clusters.jl:39) of the
-// existing nodes of one level.
+// rank[d][perm_d[r]] = r
+__global__ void k_ranks(int64_t N, const uint32_t *__restrict__ perm, uint32_t *__restrict__ rank) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    for (int d = 0; d < 3; ++d) rank[d * N + perm[d * N + r]] = (uint32_t)r;
+}
+
+// Level-0 lists: rk[(3 d + e) N + r] = r_e of the particle at rank r on axis d.
+__global__ void k_lists_init(int64_t N, const uint32_t *__restrict__ perm,
+                             const uint32_t *__restrict__ rank, uint32_t *__restrict__ rk) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    for (int d = 0; d < 3; ++d) {
+        const uint32_t p = perm[d * N + r];
+        for (int e = 0; e < 3; ++e) rk[(3 * d + e) * N + r] = e == d ? (uint32_t)r : rank[e * N + p];
+    }
+}
+
+// Bounding box (clusters.jl:51-70), split axis (clusters.jl:39) and pivot rank
+// of the existing nodes of one level; ext records the particle attaining each
+// bound (argmin/argmax, lowest id on ties).
 __global__ void k_level_nodes(int nnodes, const int32_t *__restrict__ nodes,
                               const int32_t *__restrict__ lo, const int32_t *__restrict__ cnt,
-                              const uint8_t *__restrict__ flags, const uint32_t *__restrict__ L,
-                              int64_t N, const double *__restrict__ pos,
-                              const double *__restrict__ pavg4, double *__restrict__ bbox,
-                              int8_t *__restrict__ axis, int32_t *__restrict__ ext) {
+                              const uint8_t *__restrict__ flags, const uint32_t *__restrict__ rk,
+                              const uint32_t *__restrict__ perm, int64_t N,
+                              const double *__restrict__ pos, const double *__restrict__ pavg4,
+                              double *__restrict__ bbox, int8_t *__restrict__ axis,
+                              uint32_t *__restrict__ pivot, int32_t *__restrict__ ext) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nnodes) return;
     int32_t k = nodes[t];
     int64_t a = lo[k], b = (int64_t)lo[k] + cnt[k] - 1;
     double mn[3], mx[3];
     for (int d = 0; d < 3; ++d) {
-        const uint32_t pa = L[d * N + a], pb = L[d * N + b];  // argmin / argmax particle
+        const uint32_t pa = perm[d * N + rk[(4 * d) * N + a]];  // list d, own rank
+        const uint32_t pb = perm[d * N + rk[(4 * d) * N + b]];
         mn[d] = pos[3 * (int64_t)pa + d] + 0.0;
         mx[d] = pos[3 * (int64_t)pb + d] + 0.0;
         bbox[6 * (int64_t)k + d] = mn[d];
@@ -154,91 +180,96 @@ __global__ void k_level_nodes(int nnodes, const int32_t *__restrict__ nodes,
             }
         }
         axis[k] = (int8_t)dir;
+        pivot[k] = rk[(4 * dir) * N + a + ((cnt[k] + 1) >> 1)];  // first right entry of list dir
     }
 }
 
-__global__ void k_split_flags(int64_t N, int l, const int32_t *__restrict__ seg,
-                              const uint8_t *__restrict__ flags, const int8_t *__restrict__ axis,
-                              const int32_t *__restrict__ lo, const int32_t *__restrict__ cnt,
-                              const uint32_t *__restrict__ L, uint8_t *__restrict__ flag,
-                              int32_t *__restrict__ seg_out) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    int32_t k = seg[i];
-    if (splitting(k, l, flags)) {
-        int a = axis[k];
-        int32_t nl = (cnt[k] + 1) >> 1;
-        bool left = (i - lo[k]) < nl;
-        flag[L[a * N + i]] = left ? 1 : 0;
-        seg_out[i] = left ? 2 * k + 1 : 2 * k + 2;
-    } else {
-        seg_out[i] = k;
-    }
+// Side of list entry i (list d) at level l: 1 = left, 0 = right or not splitting.
+__device__ inline uint32_t part_side(int64_t N, int l, int d, int64_t i, int32_t k,
+                                     const uint8_t *flags, const int8_t *axis,
+                                     const uint32_t *pivot, const uint32_t *rk) {
+    if (!splitting(k, l, flags)) return 0u;
+    const int a = axis[k];
+    return rk[(3 * d + a) * N + i] < pivot[k] ? 1u : 0u;
 }
 
+// Per-block counts of left entries of each list (blockIdx.y = list).
 __global__ __launch_bounds__(PT_THREADS) void k_part_count(
     int64_t N, int l, const int32_t *__restrict__ seg, const uint8_t *__restrict__ flags,
-    const uint32_t *__restrict__ L, const uint8_t *__restrict__ flag, uint8_t *__restrict__ fbuf,
-    uint32_t *__restrict__ bsum, int nblk) {
-    __shared__ uint32_t s[PT_THREADS];
+    const int8_t *__restrict__ axis, const uint32_t *__restrict__ pivot,
+    const uint32_t *__restrict__ rk, uint32_t *__restrict__ bsum, int nblk) {
+    __shared__ uint32_t s[PT_WAVES];
     const int d = blockIdx.y;
-    const int64_t base = (int64_t)blockIdx.x * PT_TILE + PT_ITEMS * threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * PT_TILE;
     uint32_t c = 0;
+#pragma unroll
     for (int j = 0; j < PT_ITEMS; ++j) {
-        int64_t i = base + j;
-        if (i < N) {
-            uint8_t f = 0;
-            if (splitting(seg[i], l, flags)) f = flag[L[d * N + i]];
-            fbuf[d * N + i] = f;
-            c += f;
-        }
+        const int64_t i = base + j * PT_THREADS + threadIdx.x;
+        uint32_t f = 0;
+        if (i < N) f = part_side(N, l, d, i, seg[i], flags, axis, pivot, rk);
+        c += (uint32_t)__popcll(__ballot(f));
     }
-    s[threadIdx.x] = c;
-    for (int off = PT_THREADS / 2; off > 0; off >>= 1) {
-        __syncthreads();
-        if (threadIdx.x < (unsigned)off) s[threadIdx.x] += s[threadIdx.x + off];
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < PT_WAVES; ++w) t += s[w];
+        bsum[(int64_t)d * nblk + blockIdx.x] = t;
     }
-    if (threadIdx.x == 0) bsum[(int64_t)d * nblk + blockIdx.x] = s[0];
 }
 
+// Stable segmented partition of the three lists (blockIdx.y = list): left
+// entries of a splitting cluster keep their order at lo + (left entries before
+// them), right entries at lo + ceil(c/2) + (right entries before them); others
+// stay.  bsum holds the exclusive prefix of the per-block counts, leftbase[k]
+// the number of left entries before cluster k.  List 0's blocks also write the
+// next level's segment ids.
 __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(
     int64_t N, int l, const int32_t *__restrict__ seg, const uint8_t *__restrict__ flags,
+    const int8_t *__restrict__ axis, const uint32_t *__restrict__ pivot,
     const int32_t *__restrict__ lo, const int32_t *__restrict__ cnt,
-    const int32_t *__restrict__ leftbase, const uint32_t *__restrict__ L,
-    const uint8_t *__restrict__ fbuf, const uint32_t *__restrict__ bsum, int nblk,
-    uint32_t *__restrict__ Lout) {
-    __shared__ uint32_t s[PT_THREADS];
+    const int32_t *__restrict__ leftbase, const uint32_t *__restrict__ rk,
+    const uint32_t *__restrict__ bsum, int nblk, uint32_t *__restrict__ rk_out,
+    int32_t *__restrict__ seg_out) {
+    __shared__ uint32_t s[PT_ITEMS][PT_WAVES];
     const int d = blockIdx.y;
-    const int64_t base = (int64_t)blockIdx.x * PT_TILE + PT_ITEMS * threadIdx.x;
-    uint8_t f[PT_ITEMS];
-    uint32_t c = 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * PT_TILE;
+    uint64_t bal[PT_ITEMS];
+#pragma unroll
     for (int j = 0; j < PT_ITEMS; ++j) {
-        int64_t i = base + j;
-        f[j] = i < N ? fbuf[d * N + i] : 0;
-        c += f[j];
+        const int64_t i = base + j * PT_THREADS + threadIdx.x;
+        uint32_t f = 0;
+        if (i < N) f = part_side(N, l, d, i, seg[i], flags, axis, pivot, rk);
+        bal[j] = __ballot(f);
+        if (lane == 0) s[j][w] = (uint32_t)__popcll(bal[j]);
     }
-    // exclusive block scan of the per-thread counts
-    s[threadIdx.x] = c;
     __syncthreads();
-    for (int off = 1; off < PT_THREADS; off <<= 1) {
-        uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
-        __syncthreads();
-        s[threadIdx.x] += t;
-        __syncthreads();
-    }
-    int64_t run = (int64_t)bsum[(int64_t)d * nblk + blockIdx.x] + s[threadIdx.x] - c;
+    const uint64_t below = (1ull << lane) - 1ull;
+    int64_t run = bsum[(int64_t)d * nblk + blockIdx.x];
+#pragma unroll
     for (int j = 0; j < PT_ITEMS; ++j) {
-        int64_t i = base + j;
-        if (i >= N) break;
-        int32_t k = seg[i];
-        int64_t np = i;
-        if (splitting(k, l, flags)) {
-            int64_t lft = run - leftbase[k];
-            int64_t nl = (cnt[k] + 1) >> 1;
-            np = f[j] ? lo[k] + lft : lo[k] + nl + (i - lo[k]) - lft;
+        const int64_t i = base + j * PT_THREADS + threadIdx.x;
+        uint32_t before = 0, total = 0;
+        for (int v = 0; v < PT_WAVES; ++v) {
+            before += v < w ? s[j][v] : 0u;
+            total += s[j][v];
         }
-        Lout[d * N + np] = L[d * N + i];
-        run += f[j];
+        if (i < N) {
+            const int32_t k = seg[i];
+            int64_t np = i;
+            if (splitting(k, l, flags)) {
+                const int64_t lft = run + before + (uint32_t)__popcll(bal[j] & below) - leftbase[k];
+                const int64_t nl = (cnt[k] + 1) >> 1;
+                np = ((bal[j] >> lane) & 1ull) ? lo[k] + lft : lo[k] + nl + (i - lo[k]) - lft;
+                if (d == 0) seg_out[i] = (i - lo[k]) < nl ? 2 * k + 1 : 2 * k + 2;
+            } else if (d == 0) {
+                seg_out[i] = k;
+            }
+#pragma unroll
+            for (int e = 0; e < 3; ++e) rk_out[(3 * d + e) * N + np] = rk[(3 * d + e) * N + i];
+        }
+        run += total;
     }
 }
 
@@ -246,13 +277,14 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(
 __global__ __launch_bounds__(256) void k_leaf_sort(const int32_t *__restrict__ leaves,
                                                    const int32_t *__restrict__ lo,
                                                    const int32_t *__restrict__ cnt,
-                                                   const uint32_t *__restrict__ Lx,
+                                                   const uint32_t *__restrict__ rkx,
+                                                   const uint32_t *__restrict__ permx,
                                                    int32_t *__restrict__ parind,
                                                    int32_t *__restrict__ leafof) {
     __shared__ uint32_t s[MAX_N0];
     const int32_t k = leaves[blockIdx.x];
     const int32_t a = lo[k], c = cnt[k];
-    for (int j = threadIdx.x; j < c; j += blockDim.x) s[j] = Lx[a + j];
+    for (int j = threadIdx.x; j < c; j += blockDim.x) s[j] = permx[rkx[a + j]];
     __syncthreads();
     for (int j = threadIdx.x; j < c; j += blockDim.x) {
         uint32_t v = s[j];
@@ -285,52 +317,56 @@ __global__ void k_gather(int64_t N, const int32_t *__restrict__ parind,
 static inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, int64_t N0) {
-    (void)mom;
     (void)N0;
     const Shape &S = c.shape;
     hipStream_t st = c.stream;
     uint64_t *keys = c.d_keys.as<uint64_t>(), *keys_alt = c.d_keys_alt.as<uint64_t>();
     uint32_t *vals = c.d_vals.as<uint32_t>(), *vals_alt = c.d_vals_alt.as<uint32_t>();
     int32_t *seg = c.d_seg.as<int32_t>(), *seg_alt = c.d_seg_alt.as<int32_t>();
+    uint32_t *rk = c.d_rk.as<uint32_t>(), *rk_alt = rk + 9 * (size_t)N;
 
     hipLaunchKernelGGL(k_make_keys, dim3(nb(N, 256)), dim3(256), 0, st, pos, N, keys, vals, seg);
     FMM_LAUNCH_CHECK();
     bool alt = radix_sort(keys, keys_alt, vals, vals_alt, N, 3, 0, 64,
                           c.d_rs_scratch.as<uint32_t>(), st);
-    uint32_t *L = alt ? vals_alt : vals, *Lo = alt ? vals : vals_alt;
+    const uint32_t *perm = alt ? vals_alt : vals;
+    uint32_t *rank = alt ? vals : vals_alt;
+    hipLaunchKernelGGL(k_ranks, dim3(nb(N, 256)), dim3(256), 0, st, N, perm, rank);
+    FMM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_lists_init, dim3(nb(N, 256)), dim3(256), 0, st, N, perm,
+                       (const uint32_t *)rank, rk);
+    FMM_LAUNCH_CHECK();
 
     const int nblk = (int)nb(N, PT_TILE);
     const int32_t *lo = c.d_lo.as<int32_t>(), *cnt = c.d_cnt.as<int32_t>();
     const uint8_t *flags = c.d_flags.as<uint8_t>();
+    int8_t *axis = c.d_axis.as<int8_t>();
+    uint32_t *pivot = c.d_pivot.as<uint32_t>();
     for (int l = 0; l <= S.ml; ++l) {
         int nnodes = (int)(S.lvl_off[l + 1] - S.lvl_off[l]);
         if (nnodes > 0) {
             hipLaunchKernelGGL(k_level_nodes, dim3(nb(nnodes, 128)), dim3(128), 0, st, nnodes,
-                               c.d_lvl_nodes.as<int32_t>() + S.lvl_off[l], lo, cnt, flags, L, N,
-                               pos, c.d_pavg.as<double>(), c.d_bbox.as<double>(),
-                               c.d_axis.as<int8_t>(), c.d_ext.as<int32_t>());
+                               c.d_lvl_nodes.as<int32_t>() + S.lvl_off[l], lo, cnt, flags, rk,
+                               perm, N, pos, c.d_pavg.as<double>(), c.d_bbox.as<double>(), axis,
+                               pivot, c.d_ext.as<int32_t>());
             FMM_LAUNCH_CHECK();
         }
         int nsplit = (int)(S.split_off[l + 1] - S.split_off[l]);
         if (l == S.ml || nsplit == 0) continue;
-        hipLaunchKernelGGL(k_split_flags, dim3(nb(N, 256)), dim3(256), 0, st, N, l, seg, flags,
-                           c.d_axis.as<int8_t>(), lo, cnt, L, c.d_flag.as<uint8_t>(), seg_alt);
-        FMM_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_part_count, dim3(nblk, 3), dim3(PT_THREADS), 0, st, N, l, seg, flags,
-                           L, c.d_flag.as<uint8_t>(), c.d_fbuf.as<uint8_t>(),
-                           c.d_bsum.as<uint32_t>(), nblk);
+                           axis, pivot, rk, c.d_bsum.as<uint32_t>(), nblk);
         FMM_LAUNCH_CHECK();
         scan_rows(c.d_bsum.as<uint32_t>(), 3, nblk, c.d_bsum.as<uint32_t>() + 3 * (size_t)nblk, st);
         hipLaunchKernelGGL(k_part_scatter, dim3(nblk, 3), dim3(PT_THREADS), 0, st, N, l, seg,
-                           flags, lo, cnt, c.d_leftbase.as<int32_t>(), L, c.d_fbuf.as<uint8_t>(),
-                           c.d_bsum.as<uint32_t>(), nblk, Lo);
+                           flags, axis, pivot, lo, cnt, c.d_leftbase.as<int32_t>(), rk,
+                           c.d_bsum.as<uint32_t>(), nblk, rk_alt, seg_alt);
         FMM_LAUNCH_CHECK();
-        std::swap(L, Lo);
+        std::swap(rk, rk_alt);
         std::swap(seg, seg_alt);
     }
     const int nleaf = (int)S.leaves.size();
     hipLaunchKernelGGL(k_leaf_sort, dim3(nleaf), dim3(256), 0, st, c.d_leaves.as<int32_t>(), lo,
-                       cnt, L, c.d_parind.as<int32_t>(), c.d_leafof.as<int32_t>());
+                       cnt, rk, perm, c.d_parind.as<int32_t>(), c.d_leafof.as<int32_t>());
     FMM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_gather, dim3(nb(N, 256)), dim3(256), 0, st, N, c.d_parind.as<int32_t>(),
                        pos, mom, c.d_sp8.as<double>(),
