@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--n", type=int, default=4)
     ap.add_argument("--N0", type=int, default=125)
     ap.add_argument("--eta", type=float, default=0.5)
+    ap.add_argument("--workload", choices=["bunch", "ellipsoid"], default="bunch",
+                    help="bunch: Gaussian, sigma=(1e-3,1e-3,1e-4), p_z~N(100,1); ellipsoid: "
+                         "uniform, semi-axes (1e-3,1e-3,1e-4), p_z~N(200,2) (the 25.6M case)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 17,
                     help="particles of the serial CPU FMM sample (0 = skip)")
     ap.add_argument("--skip-fp64", action="store_true")
@@ -111,7 +114,14 @@ def main():
 
     lam = 1e-3
     dev = torch.device("cuda:0")
-    pos, mom = W.bunch(a.N, seed=3)
+    if a.workload == "bunch":
+        pos, mom = W.bunch(a.N, seed=3)
+        desc = (f"relativistic Gaussian bunch, N={a.N}, sigma=(1e-3,1e-3,1e-4), "
+                "p_z~N(100,1), p_x,p_y~N(0,0.01), lambda=1e-3")
+    else:
+        pos, mom = W.ellipsoid(a.N, seed=5)
+        desc = (f"uniform ellipsoid, N={a.N}, semi-axes (1e-3,1e-3,1e-4), p_z~N(200,2), "
+                "p_x,p_y~N(0,0.02), lambda=1e-3")
     dpos = torch.from_numpy(pos).to(dev)
     dmom = torch.from_numpy(mom).to(dev)
     bufs_m = (dpos, dmom, torch.empty_like(dpos), torch.empty_like(dpos))
@@ -162,8 +172,7 @@ def main():
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_m * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64 (mixed)", "data": "synthetic",
-        "config": {"workload": f"relativistic Gaussian bunch, N={a.N}, sigma=(1e-3,1e-3,1e-4), "
-                               "p_z~N(100,1), p_x,p_y~N(0,0.01), lambda=1e-3",
+        "config": {"workload": desc,
                    "N": a.N, "n": a.n, "N0": a.N0, "eta": a.eta, "precision": "mixed"},
         "accuracy": acc,
         "rel_err_ok": (acc.get("mixed_vs_fp64", {"E": 0, "B": 0})["E"] <= 1e-5

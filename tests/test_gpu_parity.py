@@ -213,6 +213,17 @@ def test_op_m2l_mixed_every_order(ctx, orc, n):
     assert W.rel_l2(lem, leo) <= MIXED_TOL and W.rel_l2(lbm, lbo) <= MIXED_TOL
 
 
+def test_fp64_fmm_ellipsoid_matches_oracle(ctx, orc):
+    """The 25.6M-particle case's distribution (uniform ellipsoid, p_z ~ N(200, 2))
+    at N = 2^15: FP64 FMM vs the oracle's serial FMM, and mixed vs FP64."""
+    pos, mom = W.ellipsoid(1 << 15, seed=5)
+    E, B, st = ctx.field(pos, mom, -1.0, 1e-3, 4, 125, 0.5, "fp64")
+    Eo, Bo = orc.fmm(pos, mom, -1.0, 1e-3, 4, 125, 0.5, pavg=np.array(st.p_avg))
+    assert W.rel_l2(E, Eo) <= FP64_TOL and W.rel_l2(B, Bo) <= FP64_TOL
+    Em, Bm, _ = ctx.field(pos, mom, -1.0, 1e-3, 4, 125, 0.5, "mixed")
+    assert W.rel_l2(Em, E) <= MIXED_TOL and W.rel_l2(Bm, B) <= MIXED_TOL
+
+
 def test_host_api_overwrites(ctx):
     pos, mom = W.bunch(5000, seed=12)
     beam = F.Beam(pos=pos.T, mom=mom.T)

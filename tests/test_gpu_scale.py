@@ -99,3 +99,48 @@ def test_adjoint_fd_2e20(ctx, bunch20):
     print(f"2^20 adjoint: {len(errs)} coords, median {np.median(errs):.2e}, max {errs.max():.2e}, "
           f"skipped {skipped}; forward {fst.ms_total:.1f} ms, vjp {st.ms_total:.1f} ms")
     assert skipped <= 16 and np.median(errs) <= 1e-5 and errs.max() <= 1e-2
+
+
+def test_bunch_2e22_n6(ctx):
+    """N=2^22 bunch with n=6: mixed within 1e-5 of FP64; FP64 against GPU
+    BruteForce on 4096 random targets."""
+    pos, mom = W.bunch(1 << 22, seed=3)
+    tg = np.random.default_rng(41).choice(len(pos), 4096, replace=False)
+    E, B, st = ctx.field(pos, mom, -1.0, 1e-3, 6, 125, 0.5, "fp64")
+    Em, Bm, sm = ctx.field(pos, mom, -1.0, 1e-3, 6, 125, 0.5, "mixed")
+    Eb, Bb = ctx.bruteforce(pos, mom, -1.0, 1e-3, targets=tg)
+    ef, bf = W.rel_l2(E[tg], Eb), W.rel_l2(B[tg], Bb)
+    em, bm = W.rel_l2(Em, E), W.rel_l2(Bm, B)
+    print(f"2^22 n=6: fp64-vs-BF E {ef:.3e} B {bf:.3e}; mixed-vs-fp64 E {em:.3e} B {bm:.3e}; "
+          f"ms fp64 {st.ms_total:.1f} mixed {sm.ms_total:.1f}")
+    assert em <= 1e-5 and bm <= 1e-5
+    assert ef <= 1e-3 and bf <= 1e-3
+
+
+def test_ellipsoid_25_6M(ctx):
+    """N=25,600,000 uniform ellipsoid (semi-axes 1e-3, 1e-3, 1e-4; p_z ~ N(200, 2)):
+    FP64 and mixed forward (mixed within 1e-5 of FP64), FP64 against GPU
+    BruteForce on 1024 targets for n=4 and n=6 (the interpolation error must
+    fall with n), and one reverse pass (finite, nonzero gradients)."""
+    N = 25_600_000
+    pos, mom = W.ellipsoid(N, seed=5)
+    tg = np.random.default_rng(42).choice(N, 1024, replace=False)
+    Eb, Bb = ctx.bruteforce(pos, mom, -1.0, 1e-3, targets=tg)
+    E6, B6, s6 = ctx.field(pos, mom, -1.0, 1e-3, 6, 125, 0.5, "fp64")
+    e6 = W.rel_l2(E6[tg], Eb)
+    del E6, B6
+    E, B, st = ctx.field(pos, mom, -1.0, 1e-3, 4, 125, 0.5, "fp64")
+    Em, Bm, sm = ctx.field(pos, mom, -1.0, 1e-3, 4, 125, 0.5, "mixed")
+    ef, bf = W.rel_l2(E[tg], Eb), W.rel_l2(B[tg], Bb)
+    em, bm = W.rel_l2(Em, E), W.rel_l2(Bm, B)
+    del Em, Bm
+    w = np.random.default_rng(43).random(N)
+    gx, gp, sa = ctx.field_vjp(pos, mom, 2 * w[:, None] * E, 2 * w[:, None] * B, -1.0, 1e-3, 4,
+                               125, 0.5)
+    print(f"25.6M: fp64-vs-BF n=4 E {ef:.3e} B {bf:.3e}, n=6 E {e6:.3e}; mixed-vs-fp64 E {em:.3e} "
+          f"B {bm:.3e}; ms fp64 {st.ms_total:.1f} mixed {sm.ms_total:.1f} vjp {sa.ms_total:.1f} "
+          f"(n=6 fp64 {s6.ms_total:.1f}); leaves {st.nleaf} nm2l {st.nm2l} np2p {st.np2p}")
+    assert em <= 1e-5 and bm <= 1e-5
+    assert ef <= 5e-2 and bf <= 5e-2 and e6 <= ef / 3
+    assert np.all(np.isfinite(gx)) and np.all(np.isfinite(gp))
+    assert np.abs(gx).max() > 0 and np.abs(gp).max() > 0
