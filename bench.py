@@ -88,16 +88,29 @@ def roofline(st, stages, n, prec, N):
                       "frac": of / (oms * 1e-3) / PEAK[prec], "avg_launch_ms": oms}}
 
 
-def cpu_baseline(N, n, N0, eta):
+def cpu_baseline(N, n, N0, eta, N_full):
+    import numpy as np
+
     import oracle
     from fmm4rb_hip import workloads as W
     pos, mom = W.bunch(N, seed=2)
     t0 = time.perf_counter()
     oracle.fmm(pos, mom, -1.0, 1e-3, n, N0, eta)
     dt = time.perf_counter() - t0
+    # BruteForce (the accuracy oracle) on the host's cores: all N sources, 4096 targets
+    threads = max(1, min(16, os.cpu_count() or 1))
+    tg = np.random.default_rng(7).choice(N, 4096, replace=False)
+    t0 = time.perf_counter()
+    oracle.bruteforce(pos, mom, -1.0, 1e-3, targets=tg, nthreads=threads)
+    dtb = time.perf_counter() - t0
+    pairs_s = N * len(tg) / dtb
     return {"value": N / dt, "unit": "particles/s", "cores": 1, "kind": "port",
             "sample": f"serial FMM (C restatement of src/fmm, 1 thread) on a {N}-particle bunch "
-                      f"of the same distribution, n={n} N0={N0} eta={eta}: {dt:.1f} s"}
+                      f"of the same distribution, n={n} N0={N0} eta={eta}: {dt:.1f} s",
+            "bruteforce": {"pair_evals_per_s": pairs_s, "cores": threads, "kind": "port",
+                           "sample": f"BruteForce (C restatement of update_particles_field.jl:21-38, "
+                                     f"OpenMP) on the same {N} sources x 4096 targets: {dtb:.2f} s",
+                           "est_s_full_workload": float(N_full) * float(N_full) / pairs_s}}
 
 
 def main():
@@ -195,7 +208,7 @@ def main():
         torch.cuda.synchronize()
         out["copy_probe_bytes"] = src.numel() * 4
     if a.cpu_sample > 0:
-        out["cpu_baseline"] = cpu_baseline(a.cpu_sample, a.n, a.N0, a.eta)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_sample, a.n, a.N0, a.eta, a.N)
     print(json.dumps(out))
 
 
