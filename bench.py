@@ -150,23 +150,29 @@ def main():
                        "stages_ms": stage_avg(st_d),
                        "roofline": roofline(st_d[-1], stage_avg(st_d), a.n, "fp64", a.N)}
     if not a.skip_fp64 and not a.skip_adjoint:
-        # reverse mode (FP64) of L = sum w (|E|^2 + |B|^2): cotangents 2 w E, 2 w B
+        # reverse mode of L = sum w (|E|^2 + |B|^2): cotangents 2 w E, 2 w B (FP64 fields)
         w = torch.rand(a.N, 1, dtype=torch.float64, device=dev,
                        generator=torch.Generator(device=dev).manual_seed(40))
         Eb, Bb = 2 * w * bufs_d[2], 2 * w * bufs_d[3]
-        gpos, gmom = torch.empty_like(dpos), torch.empty_like(dpos)
-        call = lambda: ctx.field_vjp_device(a.N, dpos.data_ptr(), dmom.data_ptr(), Eb.data_ptr(),
-                                            Bb.data_ptr(), gpos.data_ptr(), gmom.data_ptr(),
-                                            -1.0, lam, a.n, a.N0, a.eta, "fp64")
-        for _ in range(a.warmup):
-            call()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sa = [call() for _ in range(a.steps)]
-        torch.cuda.synchronize()
-        ms_a = (time.perf_counter() - t0) / a.steps
-        res["adjoint"] = {"ms_per_step": ms_a * 1e3, "ratio_to_fp64_forward": ms_a / ms_d,
-                          "stages_ms": stage_avg(sa)}
+        grads = {}
+        for prec in ("fp64", "mixed"):
+            gpos, gmom = torch.empty_like(dpos), torch.empty_like(dpos)
+            call = lambda: ctx.field_vjp_device(a.N, dpos.data_ptr(), dmom.data_ptr(),
+                                                Eb.data_ptr(), Bb.data_ptr(), gpos.data_ptr(),
+                                                gmom.data_ptr(), -1.0, lam, a.n, a.N0, a.eta, prec)
+            for _ in range(a.warmup):
+                call()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sa = [call() for _ in range(a.steps)]
+            torch.cuda.synchronize()
+            ms_a = (time.perf_counter() - t0) / a.steps
+            res["adjoint_" + prec] = {"ms_per_step": ms_a * 1e3, "ratio_to_fp64_forward": ms_a / ms_d,
+                                      "stages_ms": stage_avg(sa)}
+            grads[prec] = (gpos.cpu().numpy(), gmom.cpu().numpy())
+        res["adjoint_mixed"]["vs_fp64"] = {
+            "gpos": W.rel_l2(grads["mixed"][0], grads["fp64"][0]),
+            "gmom": W.rel_l2(grads["mixed"][1], grads["fp64"][1])}
     # accuracy, outside the timed region
     Em, Bm = bufs_m[2].cpu().numpy(), bufs_m[3].cpu().numpy()
     acc = {}
@@ -195,8 +201,9 @@ def main():
         "stages_ms": stages,
         "roofline": roofline(s, stages, a.n, "mixed", a.N),
     }
-    if "adjoint" in res:
-        out["adjoint_fp64"] = res["adjoint"]
+    for k in ("adjoint_fp64", "adjoint_mixed"):
+        if k in res:
+            out[k] = res[k]
     if "fp64" in res:
         out["fp64"] = res["fp64"]
         out["speedup_vs_fp64"] = out["value"] / res["fp64"]["value"]

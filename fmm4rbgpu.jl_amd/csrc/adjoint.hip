@@ -453,27 +453,33 @@ struct AShape {
 
 // Target side: per target node a, over its sources: dl/dY_t[a] and dl/dp_avg;
 // with FWD the same pass also forms the forward M2L locals (row sums as in
-// k_m2l) and stores them in loc.
-template <int NP, bool FWD>
+// k_m2l) and stores them in loc.  With MIXED the node differences are formed in
+// FP64 and rounded once, the terms run in FP32 (v_rsq_f32) and every source
+// cluster's FP32 sums are added into FP64 (mp holds float4 multipoles then).
+template <int NP, bool FWD, bool MIXED>
 __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
     const int32_t *__restrict__ off, const uint64_t *__restrict__ keys, int sb,
-    const double *__restrict__ ncoord, const double *__restrict__ mp4,
+    const double *__restrict__ ncoord, const void *__restrict__ mpv,
     const double *__restrict__ lbar, const double *__restrict__ pavg4, double *__restrict__ Xbar,
     double *__restrict__ pbar_part, double *__restrict__ loc) {
+    using T = typename std::conditional<MIXED, float, double>::type;
+    using V4 = typename std::conditional<MIXED, float4, double4>::type;
     constexpr int NN = AShape<NP>::NN, BLK = AShape<NP>::BLK, TPT = AShape<NP>::TPT;
     constexpr int SH = (4 * NN > 3 * BLK) ? 4 * NN : 3 * BLK;
     __shared__ double shb[SH];
     __shared__ double sn[3 * NP];
-    double4 *smp = reinterpret_cast<double4 *>(shb);
+    V4 *smp = reinterpret_cast<V4 *>(shb);
     double(*yb)[3] = reinterpret_cast<double(*)[3]>(shb);
     double(*pr3)[3] = reinterpret_cast<double(*)[3]>(shb);
+    const V4 *mp = reinterpret_cast<const V4 *>(mpv);
     const int tid = threadIdx.x;
     const int32_t t = blockIdx.x;
     const int32_t beg = off[t], end = off[t + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
-    const double px = pavg4[0], py = pavg4[1], pz = pavg4[2];
-    double tx[TPT], ty[TPT], tz[TPT], e[TPT][3], b[TPT][3], Yb[TPT][3], F[TPT][6];
-    double pb[3] = {0.0, 0.0, 0.0};
+    const T px = (T)pavg4[0], py = (T)pavg4[1], pz = (T)pavg4[2];
+    double tx[TPT], ty[TPT], tz[TPT];
+    T e[TPT][3], b[TPT][3], Yb[TPT][3], F[TPT][6], pb[3] = {0, 0, 0};
+    double Yd[TPT][3], Fd[TPT][6], pd[3] = {0.0, 0.0, 0.0};  // FP64 sums (MIXED)
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
         const int node = min(tid + q * BLK, NN - 1);
@@ -482,38 +488,44 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
         tz[q] = ncoord[((int64_t)t * 3 + 2) * NP + node / (NP * NP)];
         const double *lb = lbar + ((int64_t)t * NN + node) * 6;
         for (int d = 0; d < 3; ++d) {
-            e[q][d] = lb[d];
-            b[q][d] = lb[3 + d];
-            Yb[q][d] = 0.0;
+            e[q][d] = (T)lb[d];
+            b[q][d] = (T)lb[3 + d];
+            Yb[q][d] = 0;
+            Yd[q][d] = 0.0;
         }
-        for (int d = 0; d < 6; ++d) F[q][d] = 0.0;
+        for (int d = 0; d < 6; ++d) F[q][d] = 0, Fd[q][d] = 0.0;
     }
     for (int32_t ei = beg; ei < end; ++ei) {
         const int64_t s = (int64_t)(keys[ei] & mask);
         __syncthreads();
-        for (int j = tid; j < NN; j += BLK) smp[j] = reinterpret_cast<const double4 *>(mp4)[s * NN + j];
+        for (int j = tid; j < NN; j += BLK) smp[j] = mp[s * NN + j];
         if (tid < 3 * NP) sn[tid] = ncoord[s * 3 * NP + tid];
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             if (tid + q * BLK >= NN) continue;
+            T dxv[NP];
+#pragma unroll
+            for (int si = 0; si < NP; ++si) dxv[si] = (T)(tx[q] - sn[si]);
             for (int sk = 0; sk < NP; ++sk) {
-                const double dz = tz[q] - sn[2 * NP + sk];
+                const T dz = (T)(tz[q] - sn[2 * NP + sk]);
                 for (int sj = 0; sj < NP; ++sj) {
-                    const double dy = ty[q] - sn[NP + sj];
-                    const double rrjk = fma(dy, dy, dz * dz) + EPS64;
-                    const double prjk = fma(py, dy, pz * dz);
-                    double G = 0.0, H = 0.0, Ax = 0.0, Ay = 0.0, Az = 0.0, Uy = 0.0, Uz = 0.0;
+                    const T dy = (T)(ty[q] - sn[NP + sj]);
+                    const T rrjk = fma(dy, dy, dz * dz) + (T)EPS64;
+                    const T prjk = fma(py, dy, pz * dz);
+                    T G = 0, H = 0, Ax = 0, Ay = 0, Az = 0, Uy = 0, Uz = 0;
 #pragma unroll
                     for (int si = 0; si < NP; ++si) {
-                        const double dx = tx[q] - sn[si];
-                        const double pr = fma(px, dx, prjk);
-                        const double d = fma(pr, pr, fma(dx, dx, rrjk));
-                        const double y = rsqrt_nr(d);
-                        const double y2 = y * y, y3 = y2 * y, y5 = y3 * y2;
-                        const double4 m = smp[si + NP * (sj + NP * sk)];
+                        const T dx = dxv[si];
+                        const T pr = fma(px, dx, prjk);
+                        const T d = fma(pr, pr, fma(dx, dx, rrjk));
+                        T y;
+                        if constexpr (MIXED) y = __builtin_amdgcn_rsqf(d);
+                        else y = rsqrt_nr(d);
+                        const T y2 = y * y, y3 = y2 * y;
+                        const V4 m = smp[si + NP * (sj + NP * sk)];
                         if (FWD) {  // forward row sums (see k_m2l)
-                            const double uu = y3 * dx;
+                            const T uu = y3 * dx;
                             G = fma(y3, m.x, G);
                             H = fma(uu, m.x, H);
                             Ax = fma(y3, m.y, Ax);
@@ -522,15 +534,15 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                             Uy = fma(uu, m.z, Uy);
                             Uz = fma(uu, m.w, Uz);
                         }
-                        const double u0 = fma(m.x, e[q][0], b[q][1] * m.w - b[q][2] * m.z);
-                        const double u1 = fma(m.x, e[q][1], b[q][2] * m.y - b[q][0] * m.w);
-                        const double u2 = fma(m.x, e[q][2], b[q][0] * m.z - b[q][1] * m.y);
-                        const double Ru = fma(dx, u0, fma(dy, u1, dz * u2));
-                        const double c3 = 3.0 * y5 * Ru;
+                        const T u0 = fma(m.x, e[q][0], b[q][1] * m.w - b[q][2] * m.z);
+                        const T u1 = fma(m.x, e[q][1], b[q][2] * m.y - b[q][0] * m.w);
+                        const T u2 = fma(m.x, e[q][2], b[q][0] * m.z - b[q][1] * m.y);
+                        const T Ru = fma(dx, u0, fma(dy, u1, dz * u2));
+                        const T c3 = (T)3 * (y3 * (y2 * Ru));
                         Yb[q][0] += y3 * u0 - c3 * fma(pr, px, dx);
                         Yb[q][1] += y3 * u1 - c3 * fma(pr, py, dy);
                         Yb[q][2] += y3 * u2 - c3 * fma(pr, pz, dz);
-                        const double cp = -c3 * pr;
+                        const T cp = -c3 * pr;
                         pb[0] = fma(cp, dx, pb[0]);
                         pb[1] = fma(cp, dy, pb[1]);
                         pb[2] = fma(cp, dz, pb[2]);
@@ -546,13 +558,29 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                 }
             }
         }
+        if constexpr (MIXED) {
+#pragma unroll
+            for (int q = 0; q < TPT; ++q) {
+                for (int d = 0; d < 3; ++d) Yd[q][d] += (double)Yb[q][d], Yb[q][d] = 0;
+                for (int d = 0; d < 6; ++d) Fd[q][d] += (double)F[q][d], F[q][d] = 0;
+            }
+            for (int d = 0; d < 3; ++d) pd[d] += (double)pb[d], pb[d] = 0;
+        }
+    }
+    if constexpr (!MIXED) {
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            for (int d = 0; d < 3; ++d) Yd[q][d] = Yb[q][d];
+            for (int d = 0; d < 6; ++d) Fd[q][d] = F[q][d];
+        }
+        for (int d = 0; d < 3; ++d) pd[d] = pb[d];
     }
     if (FWD) {
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             const int node = tid + q * BLK;
             if (node < NN)
-                for (int d = 0; d < 6; ++d) loc[((int64_t)t * NN + node) * 6 + d] = F[q][d];
+                for (int d = 0; d < 6; ++d) loc[((int64_t)t * NN + node) * 6 + d] = Fd[q][d];
         }
     }
     __syncthreads();
@@ -560,102 +588,124 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
     for (int q = 0; q < TPT; ++q) {
         const int node = tid + q * BLK;
         if (node < NN)
-            for (int d = 0; d < 3; ++d) yb[node][d] = Yb[q][d];
+            for (int d = 0; d < 3; ++d) yb[node][d] = Yd[q][d];
     }
     __syncthreads();
     if (tid < 3 * NP) {  // dl/dX_{t,d,m} = sum over nodes with index m on axis d
         const int d = tid / NP, mi = tid % NP;
-        double s = 0.0;
+        double sum = 0.0;
         for (int u1 = 0; u1 < NP; ++u1)
             for (int u2 = 0; u2 < NP; ++u2) {
                 int id[3];
                 id[d] = mi;
                 id[(d + 1) % 3] = u1;
                 id[(d + 2) % 3] = u2;
-                s += yb[id[0] + NP * (id[1] + NP * id[2])][d];
+                sum += yb[id[0] + NP * (id[1] + NP * id[2])][d];
             }
-        Xbar[(int64_t)t * 3 * NP + tid] += s;
+        Xbar[(int64_t)t * 3 * NP + tid] += sum;
     }
     __syncthreads();
-    for (int d = 0; d < 3; ++d) pr3[tid][d] = pb[d];
+    for (int d = 0; d < 3; ++d) pr3[tid][d] = pd[d];
     __syncthreads();
     if (tid < 3) {
-        double s = 0.0;
-        for (int j = 0; j < BLK; ++j) s += pr3[j][tid];
-        pbar_part[3 * (int64_t)t + tid] = s;
+        double sum = 0.0;
+        for (int j = 0; j < BLK; ++j) sum += pr3[j][tid];
+        pbar_part[3 * (int64_t)t + tid] = sum;
     }
 }
 
 // Source side: per source node b, over the targets that list it: Mbar_s[b]
-// (= dl/dgamma, dl/dm) and dl/dY_s[b].
-template <int NP>
+// (= dl/dgamma, dl/dm) and dl/dY_s[b].  MIXED as in k_adj_m2l_t (the target
+// clusters' local cotangents are staged as FP32).
+template <int NP, bool MIXED>
 __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
     const int32_t *__restrict__ offT, const uint64_t *__restrict__ keysT, int sb,
-    const double *__restrict__ ncoord, const double *__restrict__ mp4,
+    const double *__restrict__ ncoord, const void *__restrict__ mpv,
     const double *__restrict__ lbar, const double *__restrict__ pavg4, double *__restrict__ mbar,
     double *__restrict__ Xbar) {
+    using T = typename std::conditional<MIXED, float, double>::type;
+    using V4 = typename std::conditional<MIXED, float4, double4>::type;
     constexpr int NN = AShape<NP>::NN, BLK = AShape<NP>::BLK, TPT = AShape<NP>::TPT;
-    __shared__ double slb[NN * 6];
+    __shared__ double shl[NN * 6];
     __shared__ double tn[3 * NP];
-    double(*yb)[3] = reinterpret_cast<double(*)[3]>(slb);
+    T *slb = reinterpret_cast<T *>(shl);
+    double(*yb)[3] = reinterpret_cast<double(*)[3]>(shl);
+    const V4 *mp = reinterpret_cast<const V4 *>(mpv);
     const int tid = threadIdx.x;
     const int32_t s = blockIdx.x;
     const int32_t beg = offT[s], end = offT[s + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
-    const double px = pavg4[0], py = pavg4[1], pz = pavg4[2];
-    double sx[TPT], sy[TPT], sz[TPT], Yb[TPT][3], gb[TPT][4];
-    double4 mq[TPT];
+    const T px = (T)pavg4[0], py = (T)pavg4[1], pz = (T)pavg4[2];
+    double sx[TPT], sy[TPT], sz[TPT], Yd[TPT][3], gd[TPT][4];
+    T Yb[TPT][3], gb[TPT][4];
+    V4 mq[TPT];
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
         const int node = min(tid + q * BLK, NN - 1);
         sx[q] = ncoord[((int64_t)s * 3 + 0) * NP + node % NP];
         sy[q] = ncoord[((int64_t)s * 3 + 1) * NP + (node / NP) % NP];
         sz[q] = ncoord[((int64_t)s * 3 + 2) * NP + node / (NP * NP)];
-        mq[q] = reinterpret_cast<const double4 *>(mp4)[(int64_t)s * NN + node];
-        for (int d = 0; d < 3; ++d) Yb[q][d] = 0.0;
-        for (int d = 0; d < 4; ++d) gb[q][d] = 0.0;
+        mq[q] = mp[(int64_t)s * NN + node];
+        for (int d = 0; d < 3; ++d) Yb[q][d] = 0, Yd[q][d] = 0.0;
+        for (int d = 0; d < 4; ++d) gb[q][d] = 0, gd[q][d] = 0.0;
     }
     for (int32_t ei = beg; ei < end; ++ei) {
         const int64_t t = (int64_t)(keysT[ei] & mask);
         __syncthreads();
-        for (int j = tid; j < NN * 6; j += BLK) slb[j] = lbar[t * NN * 6 + j];
+        for (int j = tid; j < NN * 6; j += BLK) slb[j] = (T)lbar[t * NN * 6 + j];
         if (tid < 3 * NP) tn[tid] = ncoord[t * 3 * NP + tid];
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             if (tid + q * BLK >= NN) continue;
-            const double4 m = mq[q];
+            const V4 m = mq[q];
+            T dxv[NP];
+#pragma unroll
+            for (int ti = 0; ti < NP; ++ti) dxv[ti] = (T)(tn[ti] - sx[q]);
             for (int tk = 0; tk < NP; ++tk) {
-                const double dz = tn[2 * NP + tk] - sz[q];
+                const T dz = (T)(tn[2 * NP + tk] - sz[q]);
                 for (int tj = 0; tj < NP; ++tj) {
-                    const double dy = tn[NP + tj] - sy[q];
-                    const double rrjk = fma(dy, dy, dz * dz) + EPS64;
-                    const double prjk = fma(py, dy, pz * dz);
+                    const T dy = (T)(tn[NP + tj] - sy[q]);
+                    const T rrjk = fma(dy, dy, dz * dz) + (T)EPS64;
+                    const T prjk = fma(py, dy, pz * dz);
 #pragma unroll
                     for (int ti = 0; ti < NP; ++ti) {
-                        const double dx = tn[ti] - sx[q];
-                        const double pr = fma(px, dx, prjk);
-                        const double d = fma(pr, pr, fma(dx, dx, rrjk));
-                        const double y = rsqrt_nr(d);
-                        const double y2 = y * y, y3 = y2 * y, y5 = y3 * y2;
-                        const double *o = slb + 6 * (ti + NP * (tj + NP * tk));
-                        const double e0 = o[0], e1 = o[1], e2 = o[2], b0 = o[3], b1 = o[4], b2 = o[5];
+                        const T dx = dxv[ti];
+                        const T pr = fma(px, dx, prjk);
+                        const T d = fma(pr, pr, fma(dx, dx, rrjk));
+                        T y;
+                        if constexpr (MIXED) y = __builtin_amdgcn_rsqf(d);
+                        else y = rsqrt_nr(d);
+                        const T y2 = y * y, y3 = y2 * y;
+                        const T *o = slb + 6 * (ti + NP * (tj + NP * tk));
+                        const T e0 = o[0], e1 = o[1], e2 = o[2], b0 = o[3], b1 = o[4], b2 = o[5];
                         // dl/dgamma = e.K, dl/dm = K x b
                         gb[q][0] += y3 * fma(e0, dx, fma(e1, dy, e2 * dz));
                         gb[q][1] += y3 * (dy * b2 - dz * b1);
                         gb[q][2] += y3 * (dz * b0 - dx * b2);
                         gb[q][3] += y3 * (dx * b1 - dy * b0);
-                        const double u0 = fma(m.x, e0, b1 * m.w - b2 * m.z);
-                        const double u1 = fma(m.x, e1, b2 * m.y - b0 * m.w);
-                        const double u2 = fma(m.x, e2, b0 * m.z - b1 * m.y);
-                        const double Ru = fma(dx, u0, fma(dy, u1, dz * u2));
-                        const double c3 = 3.0 * y5 * Ru;
+                        const T u0 = fma(m.x, e0, b1 * m.w - b2 * m.z);
+                        const T u1 = fma(m.x, e1, b2 * m.y - b0 * m.w);
+                        const T u2 = fma(m.x, e2, b0 * m.z - b1 * m.y);
+                        const T Ru = fma(dx, u0, fma(dy, u1, dz * u2));
+                        const T c3 = (T)3 * (y3 * (y2 * Ru));
                         Yb[q][0] -= y3 * u0 - c3 * fma(pr, px, dx);
                         Yb[q][1] -= y3 * u1 - c3 * fma(pr, py, dy);
                         Yb[q][2] -= y3 * u2 - c3 * fma(pr, pz, dz);
                     }
                 }
             }
+            if constexpr (MIXED) {
+                for (int d = 0; d < 3; ++d) Yd[q][d] += (double)Yb[q][d], Yb[q][d] = 0;
+                for (int d = 0; d < 4; ++d) gd[q][d] += (double)gb[q][d], gb[q][d] = 0;
+            }
+        }
+    }
+    if constexpr (!MIXED) {
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            for (int d = 0; d < 3; ++d) Yd[q][d] = Yb[q][d];
+            for (int d = 0; d < 4; ++d) gd[q][d] = gb[q][d];
         }
     }
     __syncthreads();
@@ -664,8 +714,8 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
         const int node = tid + q * BLK;
         if (node < NN) {
             double *o = mbar + ((int64_t)s * NN + node) * 4;
-            for (int d = 0; d < 4; ++d) o[d] = gb[q][d];
-            for (int d = 0; d < 3; ++d) yb[node][d] = Yb[q][d];
+            for (int d = 0; d < 4; ++d) o[d] = gd[q][d];
+            for (int d = 0; d < 3; ++d) yb[node][d] = Yd[q][d];
         }
     }
     __syncthreads();
@@ -685,9 +735,32 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
 }
 
 // ------------------------------------------------------------- P2P adjoints
+// Both sides use the forward P2P's wave layout: 16 lanes x 4 groups, lane (li, g)
+// owns the 4 particles li + 16 q of its 64-particle slice and takes the staged
+// partners k = g (mod 4); the groups' sums meet in two xor-shuffles.  With MIXED
+// the differences R are formed in FP64 and rounded once, the kernel terms run in
+// FP32 (v_rsq_f32), y5 (R.u) is formed as y3 (y^2 (R.u)) so that no intermediate
+// overflows, and each staged tile's FP32 sums are added into FP64.  A pair with
+// R = 0 (self pair, coincident particles) contributes nothing: y is set to 0.
 constexpr int PT = 256;
+constexpr int PNG = 4;
 
-// Target side: gx_i += sum_j J^T u_ij.
+// Kernel value of one pair: y = d^-1/2 (0 for R = 0), pr = p.R, R rounded to T.
+template <typename T>
+__device__ inline T pair_geo(double R0d, double R1d, double R2d, T px, T py, T pz, T *R, T &pr) {
+    R[0] = (T)R0d;
+    R[1] = (T)R1d;
+    R[2] = (T)R2d;
+    pr = fma(px, R[0], fma(py, R[1], pz * R[2]));
+    const T d = fma(pr, pr, fma(R[0], R[0], fma(R[1], R[1], R[2] * R[2]))) + (T)EPS64;
+    T y;
+    if constexpr (sizeof(T) == 4) y = __builtin_amdgcn_rsqf(d);
+    else y = rsqrt_nr(d);
+    return (R0d == 0.0 && R1d == 0.0 && R2d == 0.0) ? (T)0 : y;
+}
+
+// Target side: gx_i += sum_j J^T u_ij,  u_ij = g_j e_i + b_i x p_j.
+template <bool MIXED>
 __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ leaves,
                                                   const int32_t *__restrict__ off,
                                                   const uint64_t *__restrict__ keys, int sb,
@@ -696,24 +769,34 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ le
                                                   const double *__restrict__ sp8,
                                                   const double *__restrict__ ob,
                                                   double *__restrict__ gx) {
-    __shared__ double4 sx[PT], sq[PT];
+    using T = typename std::conditional<MIXED, float, double>::type;
+    using V4 = typename std::conditional<MIXED, float4, double4>::type;
+    __shared__ double4 sx[PT];
+    __shared__ V4 sq[PT];
     const int BT = blockDim.x, tid = threadIdx.x;
+    const int lane = tid & 63, li = lane & 15, g = lane >> 4, wbase = tid - lane;
     const int32_t t = leaves[blockIdx.x];
     const int32_t a = lo[t], ct = cnt[t];
     const int32_t beg = off[t], end = off[t + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
     const double4 *p8 = reinterpret_cast<const double4 *>(sp8);
     for (int tb = 0; tb < ct; tb += BT) {
-        const int i = tb + tid;
-        const bool act = i < ct;
-        const int64_t ii = a + (act ? i : 0);
-        const double4 xi = p8[2 * ii];
-        double e[3], b[3];
-        for (int d = 0; d < 3; ++d) {
-            e[d] = ob[6 * ii + d];
-            b[d] = ob[6 * ii + 3 + d];
+        double xi[PNG][3], acc[PNG][3];
+        T e[PNG][3], b[PNG][3];
+#pragma unroll
+        for (int q = 0; q < PNG; ++q) {
+            const int64_t ii = a + min(tb + wbase + li + 16 * q, ct - 1);
+            const double4 x4 = p8[2 * ii];
+            xi[q][0] = x4.x;
+            xi[q][1] = x4.y;
+            xi[q][2] = x4.z;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                e[q][d] = (T)ob[6 * ii + d];
+                b[q][d] = (T)ob[6 * ii + 3 + d];
+                acc[q][d] = 0.0;
+            }
         }
-        double g0 = 0.0, g1 = 0.0, g2 = 0.0;
         for (int32_t ei = beg; ei < end; ++ei) {
             const int32_t s = (int32_t)(keys[ei] & mask);
             const int32_t sa = lo[s], cs = cnt[s];
@@ -724,39 +807,60 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ le
                     const double4 x4 = p8[2 * (int64_t)(sa + j)];
                     const double4 p4 = p8[2 * (int64_t)(sa + j) + 1];
                     sx[tid] = x4;
-                    sq[tid] = make_double4(x4.w, p4.x, p4.y, p4.z);
+                    sq[tid] = V4{(T)x4.w, (T)p4.x, (T)p4.y, (T)p4.z};
                 }
                 __syncthreads();
                 const int m = min(BT, cs - s0);
-                if (act)
-                    for (int jj = 0; jj < m; ++jj) {
-                        const double4 xs = sx[jj], q = sq[jj];
-                        const double R0 = xi.x - xs.x, R1 = xi.y - xs.y, R2 = xi.z - xs.z;
-                        if (R0 == 0.0 && R1 == 0.0 && R2 == 0.0) continue;
-                        const double pr = fma(q.y, R0, fma(q.z, R1, q.w * R2));
-                        const double d = fma(pr, pr, fma(R0, R0, fma(R1, R1, R2 * R2))) + EPS64;
-                        const double y = rsqrt_nr(d);
-                        const double y2 = y * y, y3 = y2 * y, y5 = y3 * y2;
-                        const double u0 = fma(q.x, e[0], b[1] * q.w - b[2] * q.z);
-                        const double u1 = fma(q.x, e[1], b[2] * q.y - b[0] * q.w);
-                        const double u2 = fma(q.x, e[2], b[0] * q.z - b[1] * q.y);
-                        const double Ru = fma(R0, u0, fma(R1, u1, R2 * u2));
-                        const double c3 = 3.0 * y5 * Ru;
-                        g0 += y3 * u0 - c3 * fma(pr, q.y, R0);
-                        g1 += y3 * u1 - c3 * fma(pr, q.z, R1);
-                        g2 += y3 * u2 - c3 * fma(pr, q.w, R2);
+                T part[PNG][3];
+#pragma unroll
+                for (int q = 0; q < PNG; ++q) part[q][0] = part[q][1] = part[q][2] = 0;
+                for (int k = g; k < m; k += PNG) {
+                    const double4 xs = sx[k];
+                    const V4 sqk = sq[k];  // gamma, px, py, pz
+#pragma unroll
+                    for (int q = 0; q < PNG; ++q) {
+                        T R[3], pr;
+                        const T y = pair_geo<T>(xi[q][0] - xs.x, xi[q][1] - xs.y, xi[q][2] - xs.z,
+                                                sqk.y, sqk.z, sqk.w, R, pr);
+                        const T y2 = y * y, y3 = y2 * y;
+                        const T u0 = fma(sqk.x, e[q][0], b[q][1] * sqk.w - b[q][2] * sqk.z);
+                        const T u1 = fma(sqk.x, e[q][1], b[q][2] * sqk.y - b[q][0] * sqk.w);
+                        const T u2 = fma(sqk.x, e[q][2], b[q][0] * sqk.z - b[q][1] * sqk.y);
+                        const T Ru = fma(R[0], u0, fma(R[1], u1, R[2] * u2));
+                        const T c3 = (T)3 * (y3 * (y2 * Ru));
+                        part[q][0] += y3 * u0 - c3 * fma(pr, sqk.y, R[0]);
+                        part[q][1] += y3 * u1 - c3 * fma(pr, sqk.z, R[1]);
+                        part[q][2] += y3 * u2 - c3 * fma(pr, sqk.w, R[2]);
                     }
+                }
+#pragma unroll
+                for (int q = 0; q < PNG; ++q)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) acc[q][d] += (double)part[q][d];
             }
         }
-        if (act) {
-            gx[3 * ii] += g0;
-            gx[3 * ii + 1] += g1;
-            gx[3 * ii + 2] += g2;
+#pragma unroll
+        for (int q = 0; q < PNG; ++q)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                acc[q][d] += __shfl_xor(acc[q][d], 16);
+                acc[q][d] += __shfl_xor(acc[q][d], 32);
+            }
+        double o[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            o[d] = g == 0 ? acc[0][d] : g == 1 ? acc[1][d] : g == 2 ? acc[2][d] : acc[3][d];
+        const int i = tb + wbase + li + 16 * g;
+        if (i < ct) {
+            const int64_t ii = a + i;
+            for (int d = 0; d < 3; ++d) gx[3 * ii + d] += o[d];
         }
     }
 }
 
-// Source side: gx_j -= sum_i J^T u_ij;  gp_j += (e.K) p/g + K x b + dK/dp^T u.
+// Source side: gx_j -= sum_i J^T u_ij;  gp_j += (e.K) p/g + K x b + dK/dp^T u
+// (the factor 1/g_j is applied once to the summed e.K).
+template <bool MIXED>
 __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ leaves,
                                                   const int32_t *__restrict__ offT,
                                                   const uint64_t *__restrict__ keysT, int sb,
@@ -765,21 +869,35 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
                                                   const double *__restrict__ sp8,
                                                   const double *__restrict__ ob,
                                                   double *__restrict__ gx, double *__restrict__ gp) {
-    __shared__ double4 tx[PT], te[PT], tb4[PT];
+    using T = typename std::conditional<MIXED, float, double>::type;
+    using V4 = typename std::conditional<MIXED, float4, double4>::type;
+    __shared__ double4 tx[PT];
+    __shared__ V4 te[PT], tb4[PT];
     const int BT = blockDim.x, tid = threadIdx.x;
+    const int lane = tid & 63, li = lane & 15, g = lane >> 4, wbase = tid - lane;
     const int32_t s = leaves[blockIdx.x];
     const int32_t a = lo[s], cs = cnt[s];
     const int32_t beg = offT[s], end = offT[s + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
     const double4 *p8 = reinterpret_cast<const double4 *>(sp8);
     for (int sb0 = 0; sb0 < cs; sb0 += BT) {
-        const int j = sb0 + tid;
-        const bool act = j < cs;
-        const int64_t jj = a + (act ? j : 0);
-        const double4 xs = p8[2 * jj];
-        const double4 p4 = p8[2 * jj + 1];
-        const double g = xs.w, px = p4.x, py = p4.y, pz = p4.z;
-        double gx0 = 0.0, gx1 = 0.0, gx2 = 0.0, gp0 = 0.0, gp1 = 0.0, gp2 = 0.0;
+        double xs[PNG][3], acc[PNG][7];  // gx (3), gp (3), sum e.K
+        T gs[PNG], ps[PNG][3];
+#pragma unroll
+        for (int q = 0; q < PNG; ++q) {
+            const int64_t jj = a + min(sb0 + wbase + li + 16 * q, cs - 1);
+            const double4 x4 = p8[2 * jj];
+            const double4 p4 = p8[2 * jj + 1];
+            xs[q][0] = x4.x;
+            xs[q][1] = x4.y;
+            xs[q][2] = x4.z;
+            gs[q] = (T)x4.w;
+            ps[q][0] = (T)p4.x;
+            ps[q][1] = (T)p4.y;
+            ps[q][2] = (T)p4.z;
+#pragma unroll
+            for (int d = 0; d < 7; ++d) acc[q][d] = 0.0;
+        }
         for (int32_t ei = beg; ei < end; ++ei) {
             const int32_t t = (int32_t)(keysT[ei] & mask);
             const int32_t ta = lo[t], ct = cnt[t];
@@ -789,48 +907,74 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
                 if (i < ct) {
                     const int64_t ii = ta + i;
                     tx[tid] = p8[2 * ii];
-                    te[tid] = make_double4(ob[6 * ii], ob[6 * ii + 1], ob[6 * ii + 2], 0.0);
-                    tb4[tid] = make_double4(ob[6 * ii + 3], ob[6 * ii + 4], ob[6 * ii + 5], 0.0);
+                    te[tid] = V4{(T)ob[6 * ii], (T)ob[6 * ii + 1], (T)ob[6 * ii + 2], (T)0};
+                    tb4[tid] = V4{(T)ob[6 * ii + 3], (T)ob[6 * ii + 4], (T)ob[6 * ii + 5], (T)0};
                 }
                 __syncthreads();
                 const int m = min(BT, ct - t0);
-                if (act)
-                    for (int k = 0; k < m; ++k) {
-                        const double4 xi = tx[k], e = te[k], b = tb4[k];
-                        const double R0 = xi.x - xs.x, R1 = xi.y - xs.y, R2 = xi.z - xs.z;
-                        if (R0 == 0.0 && R1 == 0.0 && R2 == 0.0) continue;
-                        const double pr = fma(px, R0, fma(py, R1, pz * R2));
-                        const double d = fma(pr, pr, fma(R0, R0, fma(R1, R1, R2 * R2))) + EPS64;
-                        const double y = rsqrt_nr(d);
-                        const double y2 = y * y, y3 = y2 * y, y5 = y3 * y2;
-                        const double u0 = fma(g, e.x, b.y * pz - b.z * py);
-                        const double u1 = fma(g, e.y, b.z * px - b.x * pz);
-                        const double u2 = fma(g, e.z, b.x * py - b.y * px);
-                        const double Ru = fma(R0, u0, fma(R1, u1, R2 * u2));
-                        const double c3 = 3.0 * y5 * Ru;
-                        gx0 -= y3 * u0 - c3 * fma(pr, px, R0);
-                        gx1 -= y3 * u1 - c3 * fma(pr, py, R1);
-                        gx2 -= y3 * u2 - c3 * fma(pr, pz, R2);
-                        const double eK = y3 * fma(e.x, R0, fma(e.y, R1, e.z * R2));
-                        const double cg = eK / g, cp = -c3 * pr;
-                        gp0 += fma(cg, px, fma(cp, R0, y3 * (R1 * b.z - R2 * b.y)));
-                        gp1 += fma(cg, py, fma(cp, R1, y3 * (R2 * b.x - R0 * b.z)));
-                        gp2 += fma(cg, pz, fma(cp, R2, y3 * (R0 * b.y - R1 * b.x)));
+                T part[PNG][7];
+#pragma unroll
+                for (int q = 0; q < PNG; ++q)
+#pragma unroll
+                    for (int d = 0; d < 7; ++d) part[q][d] = 0;
+                for (int k = g; k < m; k += PNG) {
+                    const double4 xi = tx[k];
+                    const V4 e = te[k], bb = tb4[k];
+#pragma unroll
+                    for (int q = 0; q < PNG; ++q) {
+                        const T px = ps[q][0], py = ps[q][1], pz = ps[q][2];
+                        T R[3], pr;
+                        const T y = pair_geo<T>(xi.x - xs[q][0], xi.y - xs[q][1], xi.z - xs[q][2],
+                                                px, py, pz, R, pr);
+                        const T y2 = y * y, y3 = y2 * y;
+                        const T u0 = fma(gs[q], e.x, bb.y * pz - bb.z * py);
+                        const T u1 = fma(gs[q], e.y, bb.z * px - bb.x * pz);
+                        const T u2 = fma(gs[q], e.z, bb.x * py - bb.y * px);
+                        const T Ru = fma(R[0], u0, fma(R[1], u1, R[2] * u2));
+                        const T c3 = (T)3 * (y3 * (y2 * Ru));
+                        part[q][0] -= y3 * u0 - c3 * fma(pr, px, R[0]);
+                        part[q][1] -= y3 * u1 - c3 * fma(pr, py, R[1]);
+                        part[q][2] -= y3 * u2 - c3 * fma(pr, pz, R[2]);
+                        const T cp = -c3 * pr;
+                        part[q][3] += fma(cp, R[0], y3 * (R[1] * bb.z - R[2] * bb.y));
+                        part[q][4] += fma(cp, R[1], y3 * (R[2] * bb.x - R[0] * bb.z));
+                        part[q][5] += fma(cp, R[2], y3 * (R[0] * bb.y - R[1] * bb.x));
+                        part[q][6] = fma(y3, fma(e.x, R[0], fma(e.y, R[1], e.z * R[2])), part[q][6]);
                     }
+                }
+#pragma unroll
+                for (int q = 0; q < PNG; ++q)
+#pragma unroll
+                    for (int d = 0; d < 7; ++d) acc[q][d] += (double)part[q][d];
             }
         }
-        if (act) {
-            gx[3 * jj] += gx0;
-            gx[3 * jj + 1] += gx1;
-            gx[3 * jj + 2] += gx2;
-            gp[3 * jj] += gp0;
-            gp[3 * jj + 1] += gp1;
-            gp[3 * jj + 2] += gp2;
+#pragma unroll
+        for (int q = 0; q < PNG; ++q)
+#pragma unroll
+            for (int d = 0; d < 7; ++d) {
+                acc[q][d] += __shfl_xor(acc[q][d], 16);
+                acc[q][d] += __shfl_xor(acc[q][d], 32);
+            }
+        double o[7];
+#pragma unroll
+        for (int d = 0; d < 7; ++d)
+            o[d] = g == 0 ? acc[0][d] : g == 1 ? acc[1][d] : g == 2 ? acc[2][d] : acc[3][d];
+        const int j = sb0 + wbase + li + 16 * g;
+        if (j < cs) {
+            const int64_t jj = a + j;
+            const double4 x4 = p8[2 * jj];
+            const double4 p4 = p8[2 * jj + 1];
+            const double cg = o[6] / x4.w;  // (sum e.K) / gamma_j
+            gx[3 * jj] += o[0];
+            gx[3 * jj + 1] += o[1];
+            gx[3 * jj + 2] += o[2];
+            gp[3 * jj] += fma(cg, p4.x, o[3]);
+            gp[3 * jj + 1] += fma(cg, p4.y, o[4]);
+            gp[3 * jj + 2] += fma(cg, p4.z, o[5]);
         }
     }
 }
 
-// ---------------------------------------------------- bookkeeping kernels
 __global__ void k_adj_prep(int64_t N, const int32_t *__restrict__ parind,
                            const double *__restrict__ Ebar, const double *__restrict__ Bbar,
                            double amp, double *__restrict__ ob, double *__restrict__ gx,
@@ -968,6 +1112,10 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
                   *cnt = c.d_cnt.as<int32_t>();
     const double *sp8 = c.d_sp8.as<double>(), *ncoord = c.d_ncoord.as<double>();
     double *loc = c.d_loc.as<double>(), *pbar = c.d_pbar.as<double>();
+    // mixed precision: the M2L^T kernels read the float4 copy of the multipoles
+    const bool mixed = c.precision == FMM4RB_MIXED;
+    if (mixed) launch_mp_to_f32(c);
+    const void *mpv = mixed ? (const void *)c.d_mp4f.p : (const void *)c.d_mp4.p;
 
     // 1. local cotangents (values only; they do not depend on the forward locals):
     //    L2P^T at the leaves, then L2L^T bottom-up
@@ -988,9 +1136,14 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     }
     // 2. forward M2L fused with the M2L^T target side, then the forward L2L
 #define ADJ_M2LT(NP)                                                                               \
-    hipLaunchKernelGGL((k_adj_m2l_t<NP, true>), dim3((unsigned)nc), dim3(AShape<NP>::BLK), 0, st, \
-                       c.d_m2l_off.as<int32_t>(), c.m2l_sorted, c.sb, ncoord,                      \
-                       c.d_mp4.as<double>(), lbar, c.d_pavg.as<double>(), xbar, pbar, loc)
+    if (mixed)                                                                                     \
+        hipLaunchKernelGGL((k_adj_m2l_t<NP, true, true>), dim3((unsigned)nc),                      \
+                           dim3(AShape<NP>::BLK), 0, st, c.d_m2l_off.as<int32_t>(), c.m2l_sorted,  \
+                           c.sb, ncoord, mpv, lbar, c.d_pavg.as<double>(), xbar, pbar, loc);       \
+    else                                                                                           \
+        hipLaunchKernelGGL((k_adj_m2l_t<NP, true, false>), dim3((unsigned)nc),                     \
+                           dim3(AShape<NP>::BLK), 0, st, c.d_m2l_off.as<int32_t>(), c.m2l_sorted,  \
+                           c.sb, ncoord, mpv, lbar, c.d_pavg.as<double>(), xbar, pbar, loc)
     ADJ_NP_SWITCH(c.np, ADJ_M2LT);
     FMM_LAUNCH_CHECK();
     launch_l2l(c);
@@ -1013,9 +1166,14 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     }
     // 4. M2L^T source side
 #define ADJ_M2LS(NP)                                                                              \
-    hipLaunchKernelGGL((k_adj_m2l_s<NP>), dim3((unsigned)nc), dim3(AShape<NP>::BLK), 0, st,      \
-                       c.d_m2lT_off.as<int32_t>(), c.m2lT_sorted, c.sb, ncoord,                   \
-                       c.d_mp4.as<double>(), lbar, c.d_pavg.as<double>(), mbar, xbar)
+    if (mixed)                                                                                    \
+        hipLaunchKernelGGL((k_adj_m2l_s<NP, true>), dim3((unsigned)nc), dim3(AShape<NP>::BLK), 0, \
+                           st, c.d_m2lT_off.as<int32_t>(), c.m2lT_sorted, c.sb, ncoord, mpv, lbar, \
+                           c.d_pavg.as<double>(), mbar, xbar);                                    \
+    else                                                                                          \
+        hipLaunchKernelGGL((k_adj_m2l_s<NP, false>), dim3((unsigned)nc), dim3(AShape<NP>::BLK),   \
+                           0, st, c.d_m2lT_off.as<int32_t>(), c.m2lT_sorted, c.sb, ncoord, mpv,    \
+                           lbar, c.d_pavg.as<double>(), mbar, xbar)
     ADJ_NP_SWITCH(c.np, ADJ_M2LS);
     FMM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_adj_pbar, dim3(1), dim3(256), 0, st, pbar, nc, pbar + 3 * nc);
@@ -1041,11 +1199,21 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     // 7. P2P^T
     int bt = ((S.maxleaf + 63) / 64) * 64;
     bt = std::max(64, std::min(bt, PT));
-    hipLaunchKernelGGL(k_adj_p2p_t, dim3(nleaf), dim3(bt), 0, st, leaves,
-                       c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, lo, cnt, sp8, ob, gx);
-    FMM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_adj_p2p_s, dim3(nleaf), dim3(bt), 0, st, leaves,
-                       c.d_p2pT_off.as<int32_t>(), c.p2pT_sorted, c.sb, lo, cnt, sp8, ob, gx, gp);
+    if (mixed) {
+        hipLaunchKernelGGL(k_adj_p2p_t<true>, dim3(nleaf), dim3(bt), 0, st, leaves,
+                           c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, lo, cnt, sp8, ob, gx);
+        FMM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_adj_p2p_s<true>, dim3(nleaf), dim3(bt), 0, st, leaves,
+                           c.d_p2pT_off.as<int32_t>(), c.p2pT_sorted, c.sb, lo, cnt, sp8, ob, gx,
+                           gp);
+    } else {
+        hipLaunchKernelGGL(k_adj_p2p_t<false>, dim3(nleaf), dim3(bt), 0, st, leaves,
+                           c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, lo, cnt, sp8, ob, gx);
+        FMM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_adj_p2p_s<false>, dim3(nleaf), dim3(bt), 0, st, leaves,
+                           c.d_p2pT_off.as<int32_t>(), c.p2pT_sorted, c.sb, lo, cnt, sp8, ob, gx,
+                           gp);
+    }
     FMM_LAUNCH_CHECK();
     // bounding boxes -> extreme particles, mean momentum, output order
     hipLaunchKernelGGL(k_adj_bbox, dim3(nbk(nc, 128)), dim3(128), 0, st, nc, c.np,

@@ -307,8 +307,6 @@ static void run_vjp(Ctx &c, int64_t N, const double *pos, const double *mom, dou
                     const double *Ebar, const double *Bbar, double *gpos, double *gmom,
                     fmm4rb_stats *stats) {
     validate(N, n, N0, eta, precision);
-    if (precision != FMM4RB_FP64)
-        fail(FMM4RB_EINVAL, "the adjoint is available in FMM4RB_FP64 only");
     if (!std::isfinite(charge) || !std::isfinite(lambda) || lambda == 0.0)
         fail(FMM4RB_EINVAL, "charge must be finite and lambda finite and non-zero");
     prepare(c, N, N0, n, precision);

@@ -83,7 +83,10 @@ int fmm4rb_field_device(fmm4rb_ctx *ctx, int64_t N, const double *d_pos, const d
  * interpolation nodes on the cluster bounding boxes and of the M2L kernel on
  * the mean momentum.  The tree and lists are rebuilt, not stored.  For a loss
  * L = sum_i w_i (|E_i|^2 + |B_i|^2), pass Ebar = 2 w E, Bbar = 2 w B.
- * FMM4RB_FP64 only.  stats: ms_m2l = transposed lists, local cotangents and the
+ * FMM4RB_MIXED evaluates the M2L^T and P2P^T kernel terms in FP32 (differences
+ * formed in FP64, sums added into FP64 per source block); the transfers, the
+ * interpolation-weight and bounding-box terms stay FP64.  stats: ms_m2l =
+ * transposed lists, local cotangents and the
  * forward M2L fused with the M2L^T target side plus the forward L2L; ms_p2p =
  * L2P^T/L2L^T geometry and the M2L^T source side; ms_downward = M2M^T, P2M^T,
  * P2P^T and the gradient assembly.

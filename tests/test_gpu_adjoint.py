@@ -148,12 +148,25 @@ def test_fmm_vjp_close_to_bruteforce_vjp(ctx):
     assert ex <= 2e-2 and ep <= 2e-2
 
 
-def test_vjp_deterministic_and_rejects_mixed(ctx):
-    import fmm4rb_hip as F
+def test_vjp_deterministic(ctx):
     pos, mom = W.bunch(5000, seed=63)
     Eb, Bb = np.ones_like(pos), np.ones_like(pos)
-    a = ctx.field_vjp(pos, mom, Eb, Bb, -1.0, LAM, 4, 64, 0.5)
-    b = ctx.field_vjp(pos, mom, Eb, Bb, -1.0, LAM, 4, 64, 0.5)
-    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
-    with pytest.raises(F.FMMError):
-        ctx.field_vjp(pos, mom, Eb, Bb, -1.0, LAM, 4, 64, 0.5, precision="mixed")
+    for prec in ("fp64", "mixed"):
+        a = ctx.field_vjp(pos, mom, Eb, Bb, -1.0, LAM, 4, 64, 0.5, precision=prec)
+        b = ctx.field_vjp(pos, mom, Eb, Bb, -1.0, LAM, 4, 64, 0.5, precision=prec)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("n", [2, 4, 6])
+def test_mixed_vjp_close_to_fp64(ctx, n):
+    """Mixed-precision reverse mode within 1e-5 (relative L2) of the FP64 one, for
+    the loss L = sum w (|E|^2 + |B|^2) on a 2^15 bunch."""
+    pos, mom = W.bunch(1 << 15, seed=64)
+    w = np.random.default_rng(65).random(len(pos))
+    E, B, _ = ctx.field(pos, mom, -1.0, LAM, n, 125, 0.5)
+    Eb, Bb = 2 * w[:, None] * E, 2 * w[:, None] * B
+    gx, gp, _ = ctx.field_vjp(pos, mom, Eb, Bb, -1.0, LAM, n, 125, 0.5)
+    gxm, gpm, _ = ctx.field_vjp(pos, mom, Eb, Bb, -1.0, LAM, n, 125, 0.5, precision="mixed")
+    ex, ep = W.rel_l2(gxm, gx), W.rel_l2(gpm, gp)
+    print(f"n={n}: mixed vs fp64 adjoint: pos {ex:.2e} mom {ep:.2e}")
+    assert ex <= 1e-5 and ep <= 1e-5
