@@ -473,7 +473,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
     double(*pr3)[3] = reinterpret_cast<double(*)[3]>(shb);
     const V4 *mp = reinterpret_cast<const V4 *>(mpv);
     const int tid = threadIdx.x;
-    const int32_t t = blockIdx.x;
+    const int32_t t = xcd_item(blockIdx.x, gridDim.x);
     const int32_t beg = off[t], end = off[t + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
     const T px = (T)pavg4[0], py = (T)pavg4[1], pz = (T)pavg4[2];
@@ -632,7 +632,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
     double(*yb)[3] = reinterpret_cast<double(*)[3]>(shl);
     const V4 *mp = reinterpret_cast<const V4 *>(mpv);
     const int tid = threadIdx.x;
-    const int32_t s = blockIdx.x;
+    const int32_t s = xcd_item(blockIdx.x, gridDim.x);
     const int32_t beg = offT[s], end = offT[s + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
     const T px = (T)pavg4[0], py = (T)pavg4[1], pz = (T)pavg4[2];
@@ -775,7 +775,7 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ le
     __shared__ V4 sq[PT];
     const int BT = blockDim.x, tid = threadIdx.x;
     const int lane = tid & 63, li = lane & 15, g = lane >> 4, wbase = tid - lane;
-    const int32_t t = leaves[blockIdx.x];
+    const int32_t t = leaves[xcd_item(blockIdx.x, gridDim.x)];
     const int32_t a = lo[t], ct = cnt[t];
     const int32_t beg = off[t], end = off[t + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
@@ -875,7 +875,7 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
     __shared__ V4 te[PT], tb4[PT];
     const int BT = blockDim.x, tid = threadIdx.x;
     const int lane = tid & 63, li = lane & 15, g = lane >> 4, wbase = tid - lane;
-    const int32_t s = leaves[blockIdx.x];
+    const int32_t s = leaves[xcd_item(blockIdx.x, gridDim.x)];
     const int32_t a = lo[s], cs = cnt[s];
     const int32_t beg = offT[s], end = offT[s + 1];
     const uint64_t mask = (1ull << sb) - 1ull;

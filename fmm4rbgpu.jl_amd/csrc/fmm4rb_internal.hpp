@@ -61,6 +61,23 @@ __device__ inline double rsqrt_nr(double d) {
 }
 constexpr int MAX_N0 = 1024;
 
+// Workgroup b of a launch runs on XCD b % 8 (round-robin dispatch over the
+// MI355X's 8 XCDs, each with its own L2).  The kernels that walk interaction
+// lists take work item xcd_item(b, G): within every chunk of 8 * XCD_RUN
+// workgroups, each XCD owns a run of XCD_RUN consecutive targets, whose
+// neighbours share most of their sources, so a source cluster or leaf is
+// fetched into fewer L2s (6x less HBM traffic in P2P), while the chunks keep
+// every XCD's share of each tree level (whole contiguous eighths of the grid
+// would give some XCDs the light upper levels and others the heavy deep ones).
+// A bijection on [0, G).
+constexpr int NXCD = 8, XCD_RUN = 64;
+__device__ inline int32_t xcd_item(int32_t b, int32_t G) {
+    constexpr int32_t CH = NXCD * XCD_RUN;
+    const int32_t base = (b / CH) * CH, n = min(CH, G - base), l = b - base;
+    const int32_t q = n / NXCD, r = n % NXCD, x = l % NXCD, k = l / NXCD;
+    return base + (x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k);
+}
+
 // Tree shape: a closed-form function of (N, N0) (clusters.jl:21-49 splits a
 // node of c particles into ceil(c/2) and floor(c/2) while c > N0).
 struct Shape {

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(M2LShape<NP>::BLK) void k_m2l(
     __shared__ double sn[3 * NP];
     const V4 *mp = reinterpret_cast<const V4 *>(mpv);
     const int tid = threadIdx.x;
-    const int32_t t = blockIdx.x;
+    const int32_t t = xcd_item(blockIdx.x, gridDim.x);
     const int32_t beg = off[t], end = off[t + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
     const T px = (T)pavg4[0], py = (T)pavg4[1], pz = (T)pavg4[2];

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
     constexpr int TL = 64 / NG;  // target lanes per wave
     const int lane = tid & 63, ti = lane % TL, g = lane / TL;
     const int wbase = tid - lane;  // first target of this wave within a pass
-    const int32_t t = leaves[blockIdx.x];
+    const int32_t t = leaves[xcd_item(blockIdx.x, gridDim.x)];
     const int32_t a = lo[t], ct = cnt[t];
     const int32_t beg = off[t], end = off[t + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
