@@ -22,7 +22,7 @@ PEAK = {"mixed": 157.3e12, "fp64": 78.6e12}  # MI355X FP32 / FP64 vector peaks (
 FLOPS_PER_PAIR = 40  # one kernel evaluation, P2P particle pair or M2L node pair
 # HBM bytes per launch from two rocprofv3 PMC passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000,
 # calibrated on a 1 GiB copy), N=2^22, n=4: profiles/m2l_p2p_bunch_2e22_n4_traffic.md
-TRAFFIC = {(1 << 22, 4, "mixed", "m2l"): 3.28e9, (1 << 22, 4, "mixed", "p2p"): 4.47e9,
+TRAFFIC = {(1 << 22, 4, "mixed", "m2l"): 3.26e9, (1 << 22, 4, "mixed", "p2p"): 4.49e9,
            (1 << 22, 4, "fp64", "m2l"): 6.97e9, (1 << 22, 4, "fp64", "p2p"): 2.80e9}
 
 
@@ -140,9 +140,12 @@ def main():
     bufs_m = (dpos, dmom, torch.empty_like(dpos), torch.empty_like(dpos))
     bufs_d = (dpos, dmom, torch.empty_like(dpos), torch.empty_like(dpos))
     torch.cuda.synchronize()
+    free0, res0 = torch.cuda.mem_get_info()[0], torch.cuda.memory_reserved()
+    # device memory held by the library (not by torch's allocator)
+    lib_bytes = lambda: (free0 - torch.cuda.mem_get_info()[0]) - (torch.cuda.memory_reserved() - res0)
     ctx = F.Context(0)
     ms_m, st_m = timed(ctx, torch, a.N, bufs_m, a.n, a.N0, a.eta, "mixed", a.steps, a.warmup, lam)
-    res = {}
+    res = {"library_device_bytes": {"forward_mixed": lib_bytes()}}
     if not a.skip_fp64:
         ms_d, st_d = timed(ctx, torch, a.N, bufs_d, a.n, a.N0, a.eta, "fp64", a.steps, a.warmup,
                            lam)
@@ -173,6 +176,15 @@ def main():
         res["adjoint_mixed"]["vs_fp64"] = {
             "gpos": W.rel_l2(grads["mixed"][0], grads["fp64"][0]),
             "gmom": W.rel_l2(grads["mixed"][1], grads["fp64"][1])}
+    res["library_device_bytes"]["after_reverse_mode"] = lib_bytes()
+    # host-pointer entry point (fmm4rb_field): PCIe copies of pos/mom in and E/B out
+    # included; reported beside the device-resident value, never as the value
+    Eh, Bh, _ = ctx.field(pos, mom, -1.0, lam, a.n, a.N0, a.eta, "mixed")
+    t0 = time.perf_counter()
+    Eh, Bh, _ = ctx.field(pos, mom, -1.0, lam, a.n, a.N0, a.eta, "mixed")
+    ms_h = time.perf_counter() - t0
+    res["host_pointers_mixed"] = {"ms_per_call": ms_h * 1e3, "particles_per_s": a.N / ms_h}
+    del Eh, Bh
     # accuracy, outside the timed region
     Em, Bm = bufs_m[2].cpu().numpy(), bufs_m[3].cpu().numpy()
     acc = {}
@@ -201,7 +213,7 @@ def main():
         "stages_ms": stages,
         "roofline": roofline(s, stages, a.n, "mixed", a.N),
     }
-    for k in ("adjoint_fp64", "adjoint_mixed"):
+    for k in ("adjoint_fp64", "adjoint_mixed", "host_pointers_mixed", "library_device_bytes"):
         if k in res:
             out[k] = res[k]
     if "fp64" in res:
