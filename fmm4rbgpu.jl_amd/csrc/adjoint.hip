@@ -451,6 +451,16 @@ struct AShape {
     static constexpr int TPT = (NN + BLK - 1) / BLK;
 };
 
+// M2L^T.  For one node pair (target node x with local cotangent (e, b), source
+// node y with multipole (g, m)), R = X - Y, c = d^-5/2 (R.u), u = g e + b x m:
+//   dl/dR = d^-3/2 u - 3 c (R + (p.R) p),   R.u = g (e.R) + m.(R x b) = g (e.R) + b.(m x R),
+//   dl/dg = d^-3/2 (e.R),  dl/dm = d^-3/2 (R x b),  dl/dp_avg = -3 c (p.R) R.
+// Summed over one side with the other fixed, d^-3/2 u splits into products of
+// fixed factors and kernel sums (e sum d^-3/2 g + b x sum d^-3/2 m on the target
+// side, g sum d^-3/2 e + (sum d^-3/2 b) x m on the source side), and on the
+// tensor grid of nodes every factor linear in R is hoisted per row (dy, dz fixed):
+// sum c R = (sum c dx, dy sum c, dz sum c).  ~30 vector instructions per pair.
+
 // Target side: per target node a, over its sources: dl/dY_t[a] and dl/dp_avg;
 // with FWD the same pass also forms the forward M2L locals (row sums as in
 // k_m2l) and stores them in loc.  With MIXED the node differences are formed in
@@ -466,6 +476,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
     using V4 = typename std::conditional<MIXED, float4, double4>::type;
     constexpr int NN = AShape<NP>::NN, BLK = AShape<NP>::BLK, TPT = AShape<NP>::TPT;
     constexpr int SH = (4 * NN > 3 * BLK) ? 4 * NN : 3 * BLK;
+    constexpr int NA = 17;  // accumulators: F[6], G, A[3], sum c R [3], sum c pr, sum c pr R [3]
     __shared__ double shb[SH];
     __shared__ double sn[3 * NP];
     V4 *smp = reinterpret_cast<V4 *>(shb);
@@ -478,8 +489,8 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
     const uint64_t mask = (1ull << sb) - 1ull;
     const T px = (T)pavg4[0], py = (T)pavg4[1], pz = (T)pavg4[2];
     double tx[TPT], ty[TPT], tz[TPT];
-    T e[TPT][3], b[TPT][3], Yb[TPT][3], F[TPT][6], pb[3] = {0, 0, 0};
-    double Yd[TPT][3], Fd[TPT][6], pd[3] = {0.0, 0.0, 0.0};  // FP64 sums (MIXED)
+    T e[TPT][3], b[TPT][3], ac[TPT][NA];
+    double ad[TPT][NA];  // FP64 sums (MIXED)
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
         const int node = min(tid + q * BLK, NN - 1);
@@ -490,10 +501,8 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
         for (int d = 0; d < 3; ++d) {
             e[q][d] = (T)lb[d];
             b[q][d] = (T)lb[3 + d];
-            Yb[q][d] = 0;
-            Yd[q][d] = 0.0;
         }
-        for (int d = 0; d < 6; ++d) F[q][d] = 0, Fd[q][d] = 0.0;
+        for (int k = 0; k < NA; ++k) ac[q][k] = 0, ad[q][k] = 0.0;
     }
     for (int32_t ei = beg; ei < end; ++ei) {
         const int64_t s = (int64_t)(keys[ei] & mask);
@@ -504,16 +513,24 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             if (tid + q * BLK >= NN) continue;
+            T *A = ac[q];
+            const T e0 = e[q][0], e1 = e[q][1], e2 = e[q][2];
+            const T b0 = b[q][0], b1 = b[q][1], b2 = b[q][2];
             T dxv[NP];
 #pragma unroll
             for (int si = 0; si < NP; ++si) dxv[si] = (T)(tx[q] - sn[si]);
+#pragma unroll 1
             for (int sk = 0; sk < NP; ++sk) {
                 const T dz = (T)(tz[q] - sn[2 * NP + sk]);
                 for (int sj = 0; sj < NP; ++sj) {
                     const T dy = (T)(ty[q] - sn[NP + sj]);
                     const T rrjk = fma(dy, dy, dz * dz) + (T)EPS64;
                     const T prjk = fma(py, dy, pz * dz);
+                    const T er = fma(dy, e1, dz * e2);        // e.R - dx e0
+                    const T v0 = fma(dy, b2, -(dz * b1));     // (R x b)_0
+                    const T v1r = dz * b0, v2r = dy * b0;     // (R x b)_1 + dx b2, (R x b)_2 - dx b1
                     T G = 0, H = 0, Ax = 0, Ay = 0, Az = 0, Uy = 0, Uz = 0;
+                    T C0 = 0, C1 = 0, Cp = 0, Cpx = 0;
 #pragma unroll
                     for (int si = 0; si < NP; ++si) {
                         const T dx = dxv[si];
@@ -524,63 +541,78 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                         else y = rsqrt_nr(d);
                         const T y2 = y * y, y3 = y2 * y;
                         const V4 m = smp[si + NP * (sj + NP * sk)];
+                        G = fma(y3, m.x, G);
+                        Ax = fma(y3, m.y, Ax);
+                        Ay = fma(y3, m.z, Ay);
+                        Az = fma(y3, m.w, Az);
                         if (FWD) {  // forward row sums (see k_m2l)
                             const T uu = y3 * dx;
-                            G = fma(y3, m.x, G);
                             H = fma(uu, m.x, H);
-                            Ax = fma(y3, m.y, Ax);
-                            Ay = fma(y3, m.z, Ay);
-                            Az = fma(y3, m.w, Az);
                             Uy = fma(uu, m.z, Uy);
                             Uz = fma(uu, m.w, Uz);
                         }
-                        const T u0 = fma(m.x, e[q][0], b[q][1] * m.w - b[q][2] * m.z);
-                        const T u1 = fma(m.x, e[q][1], b[q][2] * m.y - b[q][0] * m.w);
-                        const T u2 = fma(m.x, e[q][2], b[q][0] * m.z - b[q][1] * m.y);
-                        const T Ru = fma(dx, u0, fma(dy, u1, dz * u2));
-                        const T c3 = (T)3 * (y3 * (y2 * Ru));
-                        Yb[q][0] += y3 * u0 - c3 * fma(pr, px, dx);
-                        Yb[q][1] += y3 * u1 - c3 * fma(pr, py, dy);
-                        Yb[q][2] += y3 * u2 - c3 * fma(pr, pz, dz);
-                        const T cp = -c3 * pr;
-                        pb[0] = fma(cp, dx, pb[0]);
-                        pb[1] = fma(cp, dy, pb[1]);
-                        pb[2] = fma(cp, dz, pb[2]);
+                        const T v1 = fma(-dx, b2, v1r), v2 = fma(dx, b1, -v2r);
+                        const T Ru = fma(m.x, fma(dx, e0, er), fma(m.y, v0, fma(m.z, v1, m.w * v2)));
+                        const T c = y3 * (y2 * Ru);
+                        const T cp = c * pr;
+                        C0 += c;
+                        C1 = fma(c, dx, C1);
+                        Cp += cp;
+                        Cpx = fma(cp, dx, Cpx);
                     }
                     if (FWD) {
-                        F[q][0] += H;
-                        F[q][1] = fma(dy, G, F[q][1]);
-                        F[q][2] = fma(dz, G, F[q][2]);
-                        F[q][3] += dz * Ay - dy * Az;
-                        F[q][4] += Uz - dz * Ax;
-                        F[q][5] += dy * Ax - Uy;
+                        A[0] += H;
+                        A[1] = fma(dy, G, A[1]);
+                        A[2] = fma(dz, G, A[2]);
+                        A[3] += dz * Ay - dy * Az;
+                        A[4] += Uz - dz * Ax;
+                        A[5] += dy * Ax - Uy;
                     }
+                    A[6] += G;
+                    A[7] += Ax;
+                    A[8] += Ay;
+                    A[9] += Az;
+                    A[10] += C1;
+                    A[11] = fma(dy, C0, A[11]);
+                    A[12] = fma(dz, C0, A[12]);
+                    A[13] += Cp;
+                    A[14] += Cpx;
+                    A[15] = fma(dy, Cp, A[15]);
+                    A[16] = fma(dz, Cp, A[16]);
                 }
             }
-        }
-        if constexpr (MIXED) {
-#pragma unroll
-            for (int q = 0; q < TPT; ++q) {
-                for (int d = 0; d < 3; ++d) Yd[q][d] += (double)Yb[q][d], Yb[q][d] = 0;
-                for (int d = 0; d < 6; ++d) Fd[q][d] += (double)F[q][d], F[q][d] = 0;
-            }
-            for (int d = 0; d < 3; ++d) pd[d] += (double)pb[d], pb[d] = 0;
+            if constexpr (MIXED)
+                for (int k = 0; k < NA; ++k) ad[q][k] += (double)A[k], A[k] = 0;
         }
     }
-    if constexpr (!MIXED) {
+    double pd[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int q = 0; q < TPT; ++q) {
-            for (int d = 0; d < 3; ++d) Yd[q][d] = Yb[q][d];
-            for (int d = 0; d < 6; ++d) Fd[q][d] = F[q][d];
+    for (int q = 0; q < TPT; ++q) {
+        if constexpr (!MIXED)
+            for (int k = 0; k < NA; ++k) ad[q][k] = ac[q][k];
+        // dl/dX = e G + b x A - 3 (sum c R + p sum c pr);  dl/dp_avg = -3 sum c pr R
+        const double *S = ad[q];
+        const double E0 = e[q][0], E1 = e[q][1], E2 = e[q][2];
+        const double B0 = b[q][0], B1 = b[q][1], B2 = b[q][2];
+        const double p0 = pavg4[0], p1 = pavg4[1], p2 = pavg4[2];
+        const double Yx = E0 * S[6] + (B1 * S[9] - B2 * S[8]) - 3.0 * (S[10] + p0 * S[13]);
+        const double Yy = E1 * S[6] + (B2 * S[7] - B0 * S[9]) - 3.0 * (S[11] + p1 * S[13]);
+        const double Yz = E2 * S[6] + (B0 * S[8] - B1 * S[7]) - 3.0 * (S[12] + p2 * S[13]);
+        ad[q][0 + 6] = Yx;  // reuse slots for the node-coordinate gradient
+        ad[q][1 + 6] = Yy;
+        ad[q][2 + 6] = Yz;
+        if (tid + q * BLK < NN) {
+            pd[0] -= 3.0 * S[14];
+            pd[1] -= 3.0 * S[15];
+            pd[2] -= 3.0 * S[16];
         }
-        for (int d = 0; d < 3; ++d) pd[d] = pb[d];
     }
     if (FWD) {
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             const int node = tid + q * BLK;
             if (node < NN)
-                for (int d = 0; d < 6; ++d) loc[((int64_t)t * NN + node) * 6 + d] = Fd[q][d];
+                for (int d = 0; d < 6; ++d) loc[((int64_t)t * NN + node) * 6 + d] = ad[q][d];
         }
     }
     __syncthreads();
@@ -588,7 +620,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
     for (int q = 0; q < TPT; ++q) {
         const int node = tid + q * BLK;
         if (node < NN)
-            for (int d = 0; d < 3; ++d) yb[node][d] = Yd[q][d];
+            for (int d = 0; d < 3; ++d) yb[node][d] = ad[q][6 + d];
     }
     __syncthreads();
     if (tid < 3 * NP) {  // dl/dX_{t,d,m} = sum over nodes with index m on axis d
@@ -626,6 +658,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
     using T = typename std::conditional<MIXED, float, double>::type;
     using V4 = typename std::conditional<MIXED, float4, double4>::type;
     constexpr int NN = AShape<NP>::NN, BLK = AShape<NP>::BLK, TPT = AShape<NP>::TPT;
+    constexpr int NA = 14;  // gb[4], sum y3 e [3], sum y3 b [3], sum c R [3], sum c pr
     __shared__ double shl[NN * 6];
     __shared__ double tn[3 * NP];
     T *slb = reinterpret_cast<T *>(shl);
@@ -636,8 +669,9 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
     const int32_t beg = offT[s], end = offT[s + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
     const T px = (T)pavg4[0], py = (T)pavg4[1], pz = (T)pavg4[2];
-    double sx[TPT], sy[TPT], sz[TPT], Yd[TPT][3], gd[TPT][4];
-    T Yb[TPT][3], gb[TPT][4];
+    double sx[TPT], sy[TPT], sz[TPT];
+    T ac[TPT][NA];
+    double ad[TPT][NA];
     V4 mq[TPT];
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
@@ -646,8 +680,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
         sy[q] = ncoord[((int64_t)s * 3 + 1) * NP + (node / NP) % NP];
         sz[q] = ncoord[((int64_t)s * 3 + 2) * NP + node / (NP * NP)];
         mq[q] = mp[(int64_t)s * NN + node];
-        for (int d = 0; d < 3; ++d) Yb[q][d] = 0, Yd[q][d] = 0.0;
-        for (int d = 0; d < 4; ++d) gb[q][d] = 0, gd[q][d] = 0.0;
+        for (int k = 0; k < NA; ++k) ac[q][k] = 0, ad[q][k] = 0.0;
     }
     for (int32_t ei = beg; ei < end; ++ei) {
         const int64_t t = (int64_t)(keysT[ei] & mask);
@@ -658,7 +691,8 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             if (tid + q * BLK >= NN) continue;
-            const V4 m = mq[q];
+            T *A = ac[q];
+            const V4 m = mq[q];  // gamma, mx, my, mz
             T dxv[NP];
 #pragma unroll
             for (int ti = 0; ti < NP; ++ti) dxv[ti] = (T)(tn[ti] - sx[q]);
@@ -668,6 +702,10 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
                     const T dy = (T)(tn[NP + tj] - sy[q]);
                     const T rrjk = fma(dy, dy, dz * dz) + (T)EPS64;
                     const T prjk = fma(py, dy, pz * dz);
+                    const T w0 = fma(m.z, dz, -(m.w * dy));  // (m x R)_0
+                    const T w1r = -(m.y * dz), w2r = m.y * dy;
+                    T Sxe = 0, Ex = 0, Ey = 0, Ez = 0, Bx = 0, By = 0, Bz = 0, Sxby = 0, Sxbz = 0;
+                    T C0 = 0, C1 = 0, Cp = 0;
 #pragma unroll
                     for (int ti = 0; ti < NP; ++ti) {
                         const T dx = dxv[ti];
@@ -676,46 +714,62 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
                         T y;
                         if constexpr (MIXED) y = __builtin_amdgcn_rsqf(d);
                         else y = rsqrt_nr(d);
-                        const T y2 = y * y, y3 = y2 * y;
+                        const T y2 = y * y, y3 = y2 * y, uu = y3 * dx;
                         const T *o = slb + 6 * (ti + NP * (tj + NP * tk));
                         const T e0 = o[0], e1 = o[1], e2 = o[2], b0 = o[3], b1 = o[4], b2 = o[5];
-                        // dl/dgamma = e.K, dl/dm = K x b
-                        gb[q][0] += y3 * fma(e0, dx, fma(e1, dy, e2 * dz));
-                        gb[q][1] += y3 * (dy * b2 - dz * b1);
-                        gb[q][2] += y3 * (dz * b0 - dx * b2);
-                        gb[q][3] += y3 * (dx * b1 - dy * b0);
-                        const T u0 = fma(m.x, e0, b1 * m.w - b2 * m.z);
-                        const T u1 = fma(m.x, e1, b2 * m.y - b0 * m.w);
-                        const T u2 = fma(m.x, e2, b0 * m.z - b1 * m.y);
-                        const T Ru = fma(dx, u0, fma(dy, u1, dz * u2));
-                        const T c3 = (T)3 * (y3 * (y2 * Ru));
-                        Yb[q][0] -= y3 * u0 - c3 * fma(pr, px, dx);
-                        Yb[q][1] -= y3 * u1 - c3 * fma(pr, py, dy);
-                        Yb[q][2] -= y3 * u2 - c3 * fma(pr, pz, dz);
+                        Sxe = fma(uu, e0, Sxe);
+                        Ex = fma(y3, e0, Ex);
+                        Ey = fma(y3, e1, Ey);
+                        Ez = fma(y3, e2, Ez);
+                        Bx = fma(y3, b0, Bx);
+                        By = fma(y3, b1, By);
+                        Bz = fma(y3, b2, Bz);
+                        Sxby = fma(uu, b1, Sxby);
+                        Sxbz = fma(uu, b2, Sxbz);
+                        const T w1 = fma(m.w, dx, w1r), w2 = fma(-m.z, dx, w2r);
+                        const T Ru = fma(m.x, fma(dx, e0, fma(dy, e1, dz * e2)),
+                                         fma(b0, w0, fma(b1, w1, b2 * w2)));
+                        const T c = y3 * (y2 * Ru);
+                        C0 += c;
+                        C1 = fma(c, dx, C1);
+                        Cp = fma(c, pr, Cp);
                     }
+                    A[0] += fma(dy, Ey, fma(dz, Ez, Sxe));  // dl/dgamma = sum d^-3/2 e.R
+                    A[1] += dy * Bz - dz * By;              // dl/dm = sum d^-3/2 R x b
+                    A[2] += dz * Bx - Sxbz;
+                    A[3] += Sxby - dy * Bx;
+                    A[4] += Ex;
+                    A[5] += Ey;
+                    A[6] += Ez;
+                    A[7] += Bx;
+                    A[8] += By;
+                    A[9] += Bz;
+                    A[10] += C1;
+                    A[11] = fma(dy, C0, A[11]);
+                    A[12] = fma(dz, C0, A[12]);
+                    A[13] += Cp;
                 }
             }
-            if constexpr (MIXED) {
-                for (int d = 0; d < 3; ++d) Yd[q][d] += (double)Yb[q][d], Yb[q][d] = 0;
-                for (int d = 0; d < 4; ++d) gd[q][d] += (double)gb[q][d], gb[q][d] = 0;
-            }
-        }
-    }
-    if constexpr (!MIXED) {
-#pragma unroll
-        for (int q = 0; q < TPT; ++q) {
-            for (int d = 0; d < 3; ++d) Yd[q][d] = Yb[q][d];
-            for (int d = 0; d < 4; ++d) gd[q][d] = gb[q][d];
+            if constexpr (MIXED)
+                for (int k = 0; k < NA; ++k) ad[q][k] += (double)A[k], A[k] = 0;
         }
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
+        if constexpr (!MIXED)
+            for (int k = 0; k < NA; ++k) ad[q][k] = ac[q][k];
         const int node = tid + q * BLK;
         if (node < NN) {
+            const double *S = ad[q];
+            // dl/dY = -(g sum d^-3/2 e + (sum d^-3/2 b) x m) + 3 (sum c R + p sum c pr)
+            const double g = mq[q].x, m0 = mq[q].y, m1 = mq[q].z, m2 = mq[q].w;
+            const double p0 = pavg4[0], p1 = pavg4[1], p2 = pavg4[2];
             double *o = mbar + ((int64_t)s * NN + node) * 4;
-            for (int d = 0; d < 4; ++d) o[d] = gd[q][d];
-            for (int d = 0; d < 3; ++d) yb[node][d] = Yd[q][d];
+            for (int d = 0; d < 4; ++d) o[d] = S[d];
+            yb[node][0] = -(g * S[4] + (S[8] * m2 - S[9] * m1)) + 3.0 * (S[10] + p0 * S[13]);
+            yb[node][1] = -(g * S[5] + (S[9] * m0 - S[7] * m2)) + 3.0 * (S[11] + p1 * S[13]);
+            yb[node][2] = -(g * S[6] + (S[7] * m1 - S[8] * m0)) + 3.0 * (S[12] + p2 * S[13]);
         }
     }
     __syncthreads();
@@ -858,8 +912,11 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ le
     }
 }
 
-// Source side: gx_j -= sum_i J^T u_ij;  gp_j += (e.K) p/g + K x b + dK/dp^T u
-// (the factor 1/g_j is applied once to the summed e.K).
+// Source side: gx_j -= sum_i J^T u_ij;  gp_j += (e.K) p/g + K x b + dK/dp^T u.
+// With g_j, p_j fixed per thread: sum_i d^-3/2 u = g sum d^-3/2 e + (sum d^-3/2 b) x p,
+// R.u = g (e.R) + p.(R x b) (the R x b of dl/dm reused), c = d^-5/2 (R.u):
+//   gx_j = -(g E3 + B3 x p) + 3 (sum c R + p sum c pr),
+//   gp_j = (sum d^-3/2 e.R) p / g + sum d^-3/2 R x b - 3 sum c pr R.
 template <bool MIXED>
 __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ leaves,
                                                   const int32_t *__restrict__ offT,
@@ -871,21 +928,30 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
                                                   double *__restrict__ gx, double *__restrict__ gp) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using V4 = typename std::conditional<MIXED, float4, double4>::type;
+    // sums: E3 [0-2], B3 [3-5], sum c R [6-8], sum c pr [9], sum y3 e.R [10],
+    //       sum y3 R x b [11-13], sum c pr R [14-16]
+    constexpr int NA = 17;
+    // FP64 keeps 17 FP64 sums per source: 2 sources per lane (32 lanes x 2 groups)
+    constexpr int NG = MIXED ? PNG : 2, TL = 64 / NG;
     __shared__ double4 tx[PT];
     __shared__ V4 te[PT], tb4[PT];
     const int BT = blockDim.x, tid = threadIdx.x;
-    const int lane = tid & 63, li = lane & 15, g = lane >> 4, wbase = tid - lane;
+    const int lane = tid & 63, li = lane % TL, g = lane / TL, wbase = tid - lane;
     const int32_t s = leaves[xcd_item(blockIdx.x, gridDim.x)];
     const int32_t a = lo[s], cs = cnt[s];
     const int32_t beg = offT[s], end = offT[s + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
     const double4 *p8 = reinterpret_cast<const double4 *>(sp8);
     for (int sb0 = 0; sb0 < cs; sb0 += BT) {
-        double xs[PNG][3], acc[PNG][7];  // gx (3), gp (3), sum e.K
-        T gs[PNG], ps[PNG][3];
+        // FP64: the sums of the lane's NG sources.  MIXED: each staged tile's FP32
+        // partials meet across the groups (xor-shuffles) and the owning lane
+        // (group g owns source li + TL g) adds them into its 17 FP64 sums.
+        double xs[NG][3];
+        double acc[MIXED ? 1 : NG][NA];
+        T gs[NG], ps[NG][3];
 #pragma unroll
-        for (int q = 0; q < PNG; ++q) {
-            const int64_t jj = a + min(sb0 + wbase + li + 16 * q, cs - 1);
+        for (int q = 0; q < NG; ++q) {
+            const int64_t jj = a + min(sb0 + wbase + li + TL * q, cs - 1);
             const double4 x4 = p8[2 * jj];
             const double4 p4 = p8[2 * jj + 1];
             xs[q][0] = x4.x;
@@ -895,9 +961,11 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
             ps[q][0] = (T)p4.x;
             ps[q][1] = (T)p4.y;
             ps[q][2] = (T)p4.z;
-#pragma unroll
-            for (int d = 0; d < 7; ++d) acc[q][d] = 0.0;
         }
+#pragma unroll
+        for (int q = 0; q < (MIXED ? 1 : NG); ++q)
+#pragma unroll
+            for (int d = 0; d < NA; ++d) acc[q][d] = 0.0;
         for (int32_t ei = beg; ei < end; ++ei) {
             const int32_t t = (int32_t)(keysT[ei] & mask);
             const int32_t ta = lo[t], ct = cnt[t];
@@ -912,65 +980,97 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
                 }
                 __syncthreads();
                 const int m = min(BT, ct - t0);
-                T part[PNG][7];
+                T part[NG][NA];
+                if constexpr (MIXED)
 #pragma unroll
-                for (int q = 0; q < PNG; ++q)
+                    for (int q = 0; q < NG; ++q)
 #pragma unroll
-                    for (int d = 0; d < 7; ++d) part[q][d] = 0;
-                for (int k = g; k < m; k += PNG) {
+                        for (int d = 0; d < NA; ++d) part[q][d] = 0;
+                for (int k = g; k < m; k += NG) {
                     const double4 xi = tx[k];
                     const V4 e = te[k], bb = tb4[k];
 #pragma unroll
-                    for (int q = 0; q < PNG; ++q) {
+                    for (int q = 0; q < NG; ++q) {
                         const T px = ps[q][0], py = ps[q][1], pz = ps[q][2];
                         T R[3], pr;
                         const T y = pair_geo<T>(xi.x - xs[q][0], xi.y - xs[q][1], xi.z - xs[q][2],
                                                 px, py, pz, R, pr);
                         const T y2 = y * y, y3 = y2 * y;
-                        const T u0 = fma(gs[q], e.x, bb.y * pz - bb.z * py);
-                        const T u1 = fma(gs[q], e.y, bb.z * px - bb.x * pz);
-                        const T u2 = fma(gs[q], e.z, bb.x * py - bb.y * px);
-                        const T Ru = fma(R[0], u0, fma(R[1], u1, R[2] * u2));
-                        const T c3 = (T)3 * (y3 * (y2 * Ru));
-                        part[q][0] -= y3 * u0 - c3 * fma(pr, px, R[0]);
-                        part[q][1] -= y3 * u1 - c3 * fma(pr, py, R[1]);
-                        part[q][2] -= y3 * u2 - c3 * fma(pr, pz, R[2]);
-                        const T cp = -c3 * pr;
-                        part[q][3] += fma(cp, R[0], y3 * (R[1] * bb.z - R[2] * bb.y));
-                        part[q][4] += fma(cp, R[1], y3 * (R[2] * bb.x - R[0] * bb.z));
-                        part[q][5] += fma(cp, R[2], y3 * (R[0] * bb.y - R[1] * bb.x));
-                        part[q][6] = fma(y3, fma(e.x, R[0], fma(e.y, R[1], e.z * R[2])), part[q][6]);
+                        const T s0 = fma(e.x, R[0], fma(e.y, R[1], e.z * R[2]));
+                        const T v0 = fma(R[1], bb.z, -(R[2] * bb.y));
+                        const T v1 = fma(R[2], bb.x, -(R[0] * bb.z));
+                        const T v2 = fma(R[0], bb.y, -(R[1] * bb.x));
+                        const T Ru = fma(gs[q], s0, fma(px, v0, fma(py, v1, pz * v2)));
+                        const T c = y3 * (y2 * Ru);
+                        const T cp = c * pr;
+                        T *P;
+                        if constexpr (MIXED) P = part[q];
+                        else P = acc[q];  // FP64: the sums themselves
+                        P[0] = fma(y3, e.x, P[0]);
+                        P[1] = fma(y3, e.y, P[1]);
+                        P[2] = fma(y3, e.z, P[2]);
+                        P[3] = fma(y3, bb.x, P[3]);
+                        P[4] = fma(y3, bb.y, P[4]);
+                        P[5] = fma(y3, bb.z, P[5]);
+                        P[6] = fma(c, R[0], P[6]);
+                        P[7] = fma(c, R[1], P[7]);
+                        P[8] = fma(c, R[2], P[8]);
+                        P[9] += cp;
+                        P[10] = fma(y3, s0, P[10]);
+                        P[11] = fma(y3, v0, P[11]);
+                        P[12] = fma(y3, v1, P[12]);
+                        P[13] = fma(y3, v2, P[13]);
+                        P[14] = fma(cp, R[0], P[14]);
+                        P[15] = fma(cp, R[1], P[15]);
+                        P[16] = fma(cp, R[2], P[16]);
                     }
                 }
+                if constexpr (MIXED) {
 #pragma unroll
-                for (int q = 0; q < PNG; ++q)
+                    for (int q = 0; q < NG; ++q)
 #pragma unroll
-                    for (int d = 0; d < 7; ++d) acc[q][d] += (double)part[q][d];
+                        for (int d = 0; d < NA; ++d) {
+                            T v = part[q][d];
+                            if (NG >= 4) v += __shfl_xor(v, 16);
+                            v += __shfl_xor(v, 32);
+                            if (g == q) acc[0][d] += (double)v;
+                        }
+                }
             }
         }
+        double S[NA];
+        if constexpr (MIXED) {
 #pragma unroll
-        for (int q = 0; q < PNG; ++q)
+            for (int d = 0; d < NA; ++d) S[d] = acc[0][d];
+        } else {
 #pragma unroll
-            for (int d = 0; d < 7; ++d) {
-                acc[q][d] += __shfl_xor(acc[q][d], 16);
-                acc[q][d] += __shfl_xor(acc[q][d], 32);
+            for (int q = 0; q < NG; ++q)
+#pragma unroll
+                for (int d = 0; d < NA; ++d) {
+                    if (NG >= 4) acc[q][d] += __shfl_xor(acc[q][d], 16);
+                    acc[q][d] += __shfl_xor(acc[q][d], 32);
+                }
+#pragma unroll
+            for (int d = 0; d < NA; ++d) {
+                S[d] = acc[0][d];
+#pragma unroll
+                for (int q = 1; q < NG; ++q)
+                    if (g == q) S[d] = acc[q][d];
             }
-        double o[7];
-#pragma unroll
-        for (int d = 0; d < 7; ++d)
-            o[d] = g == 0 ? acc[0][d] : g == 1 ? acc[1][d] : g == 2 ? acc[2][d] : acc[3][d];
-        const int j = sb0 + wbase + li + 16 * g;
+        }
+        const int j = sb0 + wbase + li + TL * g;
         if (j < cs) {
             const int64_t jj = a + j;
             const double4 x4 = p8[2 * jj];
             const double4 p4 = p8[2 * jj + 1];
-            const double cg = o[6] / x4.w;  // (sum e.K) / gamma_j
-            gx[3 * jj] += o[0];
-            gx[3 * jj + 1] += o[1];
-            gx[3 * jj + 2] += o[2];
-            gp[3 * jj] += fma(cg, p4.x, o[3]);
-            gp[3 * jj + 1] += fma(cg, p4.y, o[4]);
-            gp[3 * jj + 2] += fma(cg, p4.z, o[5]);
+            const double gj = x4.w, p0 = p4.x, p1 = p4.y, p2 = p4.z;
+            const double cg = S[10] / gj;  // (sum d^-3/2 e.R) / gamma_j
+            gx[3 * jj] += -(gj * S[0] + (S[4] * p2 - S[5] * p1)) + 3.0 * (S[6] + p0 * S[9]);
+            gx[3 * jj + 1] += -(gj * S[1] + (S[5] * p0 - S[3] * p2)) + 3.0 * (S[7] + p1 * S[9]);
+            gx[3 * jj + 2] += -(gj * S[2] + (S[3] * p1 - S[4] * p0)) + 3.0 * (S[8] + p2 * S[9]);
+            gp[3 * jj] += fma(cg, p0, S[11]) - 3.0 * S[14];
+            gp[3 * jj + 1] += fma(cg, p1, S[12]) - 3.0 * S[15];
+            gp[3 * jj + 2] += fma(cg, p2, S[13]) - 3.0 * S[16];
         }
     }
 }
