@@ -132,9 +132,12 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
                                                  double *__restrict__ gp,
                                                  double *__restrict__ Xbar) {
     constexpr int NN = NP * NP * NP, NC = MODE == 0 ? 6 : 4, TPT = (NN + AT - 1) / AT;
+    // the leaf's NN x NC coefficients are read by every particle: staged in LDS (n <= 5)
+    constexpr bool STAGE = NN * NC * 8 <= 8192;
     __shared__ double nd[3 * NP];
     __shared__ double red[AT][3 * NP];  // Xbar partials, then the weight tiles
     __shared__ double Ot[AT][6];
+    __shared__ double sA[STAGE ? NN * NC : 1];
     double(*Wt)[3 * NP] = red;
     const int tid = threadIdx.x;
     const int32_t k = leaves[blockIdx.x];
@@ -144,6 +147,10 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
 #pragma unroll
     for (int j = 0; j < 3 * NP; ++j) xbl[j] = 0.0;
     const double *Ak = A + (int64_t)k * NN * NC;
+    if (STAGE && (PART & 1)) {
+        for (int j = tid; j < NN * NC; j += AT) sA[j] = Ak[j];
+        Ak = sA;
+    }
     __syncthreads();
     // per-particle gradient terms
     for (int p = tid; (PART & 1) && p < c; p += AT) {
