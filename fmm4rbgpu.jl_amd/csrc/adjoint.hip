@@ -25,8 +25,9 @@
 // side and source side), then node-coordinate gradients -> bounding boxes ->
 // the extreme particles, and the mean-momentum gradient to every particle.
 // Every accumulation has one owner per output (no atomics): results are
-// bit-reproducible.  A self pair or coincident particles (R = 0) contribute
-// nothing (K = 0 identically).
+// bit-reproducible.  The self pair (i = j) contributes nothing (K(x_i - x_i) = 0
+// for every x_i); coincident distinct particles (R = 0, i != j) contribute the
+// derivative of the regularised kernel there, d^-3/2(eps) u.
 #include "fmm4rb_internal.hpp"
 
 namespace fmm4rb {
@@ -268,10 +269,14 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
                                                      double *__restrict__ out,
                                                      double *__restrict__ Xbar) {
     constexpr int NN = NP * NP * NP, NC = MODE == 0 ? 6 : 4, TPT = (NN + AT - 1) / AT;
+    // components handled together per pass (LDS: 4 arrays of CG x NN doubles <= 48 KB)
+    constexpr int CG0 = (48 * 1024) / (4 * NN * 8);
+    constexpr int CG = CG0 >= NC ? NC : (CG0 >= 2 ? 2 : 1);
+    static_assert(NC % CG == 0, "component groups must tile the components");
     __shared__ double Om[3][NP][NP], dOm[3][NP][NP];  // [axis][child node a][parent node b]
     __shared__ int fl[3][NP];
     __shared__ double pn[3 * NP], cn[3 * NP];
-    __shared__ double bU[NN], bV[NN], t1[NN], t2[NN];
+    __shared__ double bU[CG][NN], bV[CG][NN], t1[CG][NN], t2[CG][NN];
     __shared__ double red[AT][3 * NP];
     const int tid = threadIdx.x;
     const int32_t P = parents[blockIdx.x];
@@ -302,64 +307,76 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
         for (int q = 0; q < TPT; ++q)
 #pragma unroll
             for (int m = 0; m < NC; ++m) accC[q][m] = 0.0;
-        for (int comp = 0; comp < NC; ++comp) {
+        for (int c0 = 0; c0 < NC; c0 += CG) {
             __syncthreads();
-            for (int e = tid; e < NN; e += AT) {
-                bU[e] = MODE == 0 ? Uarr[((int64_t)C * NN + e) * NC + comp]
-                                  : Uarr[((int64_t)C * NN + e) * NC + comp];
-                bV[e] = Varr[((int64_t)P * NN + e) * NC + comp];
+            for (int e = tid; e < CG * NN; e += AT) {
+                const int cc = e / NN, n = e % NN;
+                bU[cc][n] = Uarr[((int64_t)C * NN + n) * NC + c0 + cc];
+                bV[cc][n] = Varr[((int64_t)P * NN + n) * NC + c0 + cc];
             }
             __syncthreads();
             // ---- value propagation
             if (!(PART & 2)) {
             } else if (MODE == 0) {  // Lbar_P[b] += sum_a Om(a,b) Lbar_c[a]: contract a_x, a_y, a_z
-                for (int e = tid; e < NN; e += AT) {  // t1[bx, ay, az]
-                    const int bx = e % NP, r = e / NP;
+                for (int e = tid; e < CG * NN; e += AT) {  // t1[bx, ay, az]
+                    const int cc = e / NN, n = e % NN, bx = n % NP, r = n / NP;
                     double s = 0.0;
-                    for (int ax = 0; ax < NP; ++ax) s += Om[0][ax][bx] * bU[ax + NP * r];
-                    t1[e] = s;
+                    for (int ax = 0; ax < NP; ++ax) s += Om[0][ax][bx] * bU[cc][ax + NP * r];
+                    t1[cc][n] = s;
                 }
                 __syncthreads();
-                for (int e = tid; e < NN; e += AT) {  // t2[bx, by, az]
-                    const int bx = e % NP, by = (e / NP) % NP, az = e / (NP * NP);
+                for (int e = tid; e < CG * NN; e += AT) {  // t2[bx, by, az]
+                    const int cc = e / NN, n = e % NN;
+                    const int bx = n % NP, by = (n / NP) % NP, az = n / (NP * NP);
                     double s = 0.0;
-                    for (int ay = 0; ay < NP; ++ay) s += Om[1][ay][by] * t1[bx + NP * (ay + NP * az)];
-                    t2[e] = s;
+                    for (int ay = 0; ay < NP; ++ay)
+                        s += Om[1][ay][by] * t1[cc][bx + NP * (ay + NP * az)];
+                    t2[cc][n] = s;
                 }
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < TPT; ++q) {
-                    const int e = tid + q * AT;
-                    if (e < NN) {
-                        const int ij = e % (NP * NP), bz = e / (NP * NP);
-                        double s = 0.0;
-                        for (int az = 0; az < NP; ++az) s += Om[2][az][bz] * t2[ij + NP * NP * az];
-                        accP[q][comp] += s;
+                    const int n = tid + q * AT;
+                    if (n < NN) {
+                        const int ij = n % (NP * NP), bz = n / (NP * NP);
+#pragma unroll
+                        for (int cc = 0; cc < CG; ++cc) {
+                            double s = 0.0;
+                            for (int az = 0; az < NP; ++az)
+                                s += Om[2][az][bz] * t2[cc][ij + NP * NP * az];
+                            accP[q][c0 + cc] += s;
+                        }
                     }
                 }
             } else {  // Mbar_c[a] += sum_b Om(a,b) Mbar_P[b]: contract b_x, b_y, b_z
-                for (int e = tid; e < NN; e += AT) {  // t1[ax, by, bz]
-                    const int ax = e % NP, r = e / NP;
+                for (int e = tid; e < CG * NN; e += AT) {  // t1[ax, by, bz]
+                    const int cc = e / NN, n = e % NN, ax = n % NP, r = n / NP;
                     double s = 0.0;
-                    for (int bx = 0; bx < NP; ++bx) s += Om[0][ax][bx] * bV[bx + NP * r];
-                    t1[e] = s;
+                    for (int bx = 0; bx < NP; ++bx) s += Om[0][ax][bx] * bV[cc][bx + NP * r];
+                    t1[cc][n] = s;
                 }
                 __syncthreads();
-                for (int e = tid; e < NN; e += AT) {  // t2[ax, ay, bz]
-                    const int ax = e % NP, ay = (e / NP) % NP, bz = e / (NP * NP);
+                for (int e = tid; e < CG * NN; e += AT) {  // t2[ax, ay, bz]
+                    const int cc = e / NN, n = e % NN;
+                    const int ax = n % NP, ay = (n / NP) % NP, bz = n / (NP * NP);
                     double s = 0.0;
-                    for (int by = 0; by < NP; ++by) s += Om[1][ay][by] * t1[ax + NP * (by + NP * bz)];
-                    t2[e] = s;
+                    for (int by = 0; by < NP; ++by)
+                        s += Om[1][ay][by] * t1[cc][ax + NP * (by + NP * bz)];
+                    t2[cc][n] = s;
                 }
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < TPT; ++q) {
-                    const int e = tid + q * AT;
-                    if (e < NN) {
-                        const int ij = e % (NP * NP), az = e / (NP * NP);
-                        double s = 0.0;
-                        for (int bz = 0; bz < NP; ++bz) s += Om[2][az][bz] * t2[ij + NP * NP * bz];
-                        accC[q][comp] += s;
+                    const int n = tid + q * AT;
+                    if (n < NN) {
+                        const int ij = n % (NP * NP), az = n / (NP * NP);
+#pragma unroll
+                        for (int cc = 0; cc < CG; ++cc) {
+                            double s = 0.0;
+                            for (int bz = 0; bz < NP; ++bz)
+                                s += Om[2][az][bz] * t2[cc][ij + NP * NP * bz];
+                            accC[q][c0 + cc] += s;
+                        }
                     }
                 }
             }
@@ -368,26 +385,28 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
                 // V'[..] = V contracted over the two other axes with Om (b -> a)
                 const int d1 = (ax3 + 1) % 3, d2 = (ax3 + 2) % 3;
                 __syncthreads();
-                for (int e = tid; e < NN; e += AT) {  // contract axis d1: index b_d1 -> a_d1
-                    int id[3] = {e % NP, (e / NP) % NP, e / (NP * NP)};
+                for (int e = tid; e < CG * NN; e += AT) {  // contract axis d1: index b_d1 -> a_d1
+                    const int cc = e / NN, n = e % NN;
+                    int id[3] = {n % NP, (n / NP) % NP, n / (NP * NP)};
                     const int a1 = id[d1];
                     double s = 0.0;
                     for (int b1 = 0; b1 < NP; ++b1) {
                         id[d1] = b1;
-                        s += Om[d1][a1][b1] * bV[id[0] + NP * (id[1] + NP * id[2])];
+                        s += Om[d1][a1][b1] * bV[cc][id[0] + NP * (id[1] + NP * id[2])];
                     }
-                    t1[e] = s;
+                    t1[cc][n] = s;
                 }
                 __syncthreads();
-                for (int e = tid; e < NN; e += AT) {  // contract axis d2
-                    int id[3] = {e % NP, (e / NP) % NP, e / (NP * NP)};
+                for (int e = tid; e < CG * NN; e += AT) {  // contract axis d2
+                    const int cc = e / NN, n = e % NN;
+                    int id[3] = {n % NP, (n / NP) % NP, n / (NP * NP)};
                     const int a2 = id[d2];
                     double s = 0.0;
                     for (int b2 = 0; b2 < NP; ++b2) {
                         id[d2] = b2;
-                        s += Om[d2][a2][b2] * t1[id[0] + NP * (id[1] + NP * id[2])];
+                        s += Om[d2][a2][b2] * t1[cc][id[0] + NP * (id[1] + NP * id[2])];
                     }
-                    t2[e] = s;
+                    t2[cc][n] = s;
                 }
                 __syncthreads();
                 if (own) {  // gO[ax3] += sum over the other a indices of U[a] t2[(a with a_ax3 -> b)]
@@ -399,8 +418,10 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
                             ib[ax3] = pb;
                             ia[d1] = ib[d1] = u1;
                             ia[d2] = ib[d2] = u2;
-                            s += bU[ia[0] + NP * (ia[1] + NP * ia[2])] *
-                                 t2[ib[0] + NP * (ib[1] + NP * ib[2])];
+                            const int na = ia[0] + NP * (ia[1] + NP * ia[2]);
+                            const int nb = ib[0] + NP * (ib[1] + NP * ib[2]);
+#pragma unroll
+                            for (int cc = 0; cc < CG; ++cc) s += bU[cc][na] * t2[cc][nb];
                         }
                     gO[ax3] += s;
                 }
@@ -430,7 +451,6 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
             const double gxc = axis_adj<NP>(cn[tid], pn + d * NP, &Om[d][ai][0], &dOm[d][ai][0],
                                             fl[d][ai], cvec, xP);
             Xbar[(int64_t)C * 3 * NP + tid] += gxc;
-            // stash dl/dX_P of this child row in red[...][3 + ...] below
             for (int m = 0; m < NP; ++m) xbP[d * NP + m] += xP[m];
         }
     }
@@ -801,14 +821,15 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
 // partners k = g (mod 4); the groups' sums meet in two xor-shuffles.  With MIXED
 // the differences R are formed in FP64 and rounded once, the kernel terms run in
 // FP32 (v_rsq_f32), y5 (R.u) is formed as y3 (y^2 (R.u)) so that no intermediate
-// overflows, and each staged tile's FP32 sums are added into FP64.  A pair with
-// R = 0 (self pair, coincident particles) contributes nothing: y is set to 0.
+// overflows, and each staged tile's FP32 sums are added into FP64.  The self
+// pair (same particle) contributes nothing: y is set to 0.
 constexpr int PT = 256;
 constexpr int PNG = 4;
 
-// Kernel value of one pair: y = d^-1/2 (0 for R = 0), pr = p.R, R rounded to T.
+// Kernel value of one pair: y = d^-1/2 (0 for the self pair), pr = p.R, R rounded to T.
 template <typename T>
-__device__ inline T pair_geo(double R0d, double R1d, double R2d, T px, T py, T pz, T *R, T &pr) {
+__device__ inline T pair_geo(double R0d, double R1d, double R2d, T px, T py, T pz, bool self,
+                             T *R, T &pr) {
     R[0] = (T)R0d;
     R[1] = (T)R1d;
     R[2] = (T)R2d;
@@ -817,7 +838,7 @@ __device__ inline T pair_geo(double R0d, double R1d, double R2d, T px, T py, T p
     T y;
     if constexpr (sizeof(T) == 4) y = __builtin_amdgcn_rsqf(d);
     else y = rsqrt_nr(d);
-    return (R0d == 0.0 && R1d == 0.0 && R2d == 0.0) ? (T)0 : y;
+    return self ? (T)0 : y;
 }
 
 // Target side: gx_i += sum_j J^T u_ij,  u_ij = g_j e_i + b_i x p_j.
@@ -881,8 +902,9 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ le
 #pragma unroll
                     for (int q = 0; q < PNG; ++q) {
                         T R[3], pr;
+                        const bool self = t == s && tb + wbase + li + 16 * q == s0 + k;
                         const T y = pair_geo<T>(xi[q][0] - xs.x, xi[q][1] - xs.y, xi[q][2] - xs.z,
-                                                sqk.y, sqk.z, sqk.w, R, pr);
+                                                sqk.y, sqk.z, sqk.w, self, R, pr);
                         const T y2 = y * y, y3 = y2 * y;
                         const T u0 = fma(sqk.x, e[q][0], b[q][1] * sqk.w - b[q][2] * sqk.z);
                         const T u1 = fma(sqk.x, e[q][1], b[q][2] * sqk.y - b[q][0] * sqk.w);
@@ -1000,8 +1022,9 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
                     for (int q = 0; q < NG; ++q) {
                         const T px = ps[q][0], py = ps[q][1], pz = ps[q][2];
                         T R[3], pr;
+                        const bool self = s == t && sb0 + wbase + li + TL * q == t0 + k;
                         const T y = pair_geo<T>(xi.x - xs[q][0], xi.y - xs[q][1], xi.z - xs[q][2],
-                                                px, py, pz, R, pr);
+                                                px, py, pz, self, R, pr);
                         const T y2 = y * y, y3 = y2 * y;
                         const T s0 = fma(e.x, R[0], fma(e.y, R[1], e.z * R[2]));
                         const T v0 = fma(R[1], bb.z, -(R[2] * bb.y));

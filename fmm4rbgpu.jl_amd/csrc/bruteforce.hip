@@ -67,7 +67,8 @@ __global__ __launch_bounds__(BF_T) void k_bruteforce(const double *__restrict__ 
 
 // Reverse mode of BruteForce: for every ordered pair (i, j), u = g_j e_i + b_i x p_j,
 // G = d^-3/2 u - 3 d^-5/2 (R.u)(R + (p_j.R) p_j); gpos_i += G, gpos_j -= G,
-// gmom_j += (e_i.K) p_j/g_j + K x b_i - 3 d^-5/2 (p_j.R)(R.u) R.  R = 0 pairs add nothing.
+// gmom_j += (e_i.K) p_j/g_j + K x b_i - 3 d^-5/2 (p_j.R)(R.u) R.  The self pair adds
+// nothing; coincident distinct particles add the regularised kernel's derivative.
 // ob holds amp * (Ebar, Bbar) per particle (original order).  Target side:
 __global__ __launch_bounds__(BF_T) void k_bf_vjp_t(const double *__restrict__ pos,
                                                    const double *__restrict__ mom, int64_t N,
@@ -99,8 +100,8 @@ __global__ __launch_bounds__(BF_T) void k_bf_vjp_t(const double *__restrict__ po
         if (act)
             for (int jj = 0; jj < m; ++jj) {
                 const double *t = tile[jj];
+                if (j0 + jj == i) continue;  // self pair
                 const double R0 = x0 - t[0], R1 = x1 - t[1], R2 = x2 - t[2];
-                if (R0 == 0.0 && R1 == 0.0 && R2 == 0.0) continue;
                 const double px = t[4], py = t[5], pz = t[6], g = t[3];
                 const double pr = (px * R0 + py * R1) + pz * R2;
                 const double d = ((R0 * R0 + R1 * R1) + R2 * R2) + pr * pr + EPS64;
@@ -149,8 +150,8 @@ __global__ __launch_bounds__(BF_T) void k_bf_vjp_s(const double *__restrict__ po
         if (act)
             for (int k = 0; k < m; ++k) {
                 const double *t = tile[k];
+                if (i0 + k == j) continue;  // self pair
                 const double R0 = t[0] - x0, R1 = t[1] - x1, R2 = t[2] - x2;
-                if (R0 == 0.0 && R1 == 0.0 && R2 == 0.0) continue;
                 const double e0 = t[3], e1 = t[4], e2 = t[5], b0 = t[6], b1 = t[7], b2 = t[8];
                 const double pr = (px * R0 + py * R1) + pz * R2;
                 const double d = ((R0 * R0 + R1 * R1) + R2 * R2) + pr * pr + EPS64;

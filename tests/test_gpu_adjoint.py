@@ -51,6 +51,33 @@ def test_bruteforce_vjp_matches_finite_differences(ctx):
     assert np.median(errs) <= 1e-7 and max(errs) <= 1e-4, errs
 
 
+def test_coincident_particles_vjp(ctx):
+    """Two distinct particles at the same position: the regularised kernel is smooth
+    at R = 0 (K(R) ~ R eps^-3/2), so its derivative there enters the gradient; only the
+    self pair is skipped.  BruteForce VJP vs FD on the duplicated coordinates, and the
+    FMM's P2P-only VJP (eta = 0) equal to it."""
+    pos, mom = W.bunch(300, seed=70)
+    pos[1] = pos[0]
+    w = np.random.default_rng(71).random(len(pos))
+    E, B = ctx.bruteforce(pos, mom, -1.0, LAM)
+    Eb, Bb = 2 * w[:, None] * E, 2 * w[:, None] * B
+    gx, gp = ctx.bruteforce_vjp(pos, mom, Eb, Bb, -1.0, LAM)
+    fields = lambda x, p: ctx.bruteforce(x, p, -1.0, LAM)
+    # around R = 0 the kernel varies on the scale sqrt(eps) / |p| ~ 1.5e-10: position
+    # steps of the duplicated pair must be far below it to see the derivative
+    hs = (1e-14, _steps(pos, mom)[1])
+    errs = []
+    for i, a, d in [(0, 0, 0), (0, 0, 2), (1, 0, 1), (1, 0, 2), (0, 1, 0), (1, 1, 2)]:
+        g = (gx, gp)[a][i, d]
+        fd = _fd(fields, w, pos, mom, i, a, d, hs[a])
+        errs.append(abs(g - fd) / (abs(fd) + 1e-3 * np.sqrt(np.mean((gx, gp)[a] ** 2))))
+    assert max(errs) <= 1e-4, errs
+    Ef, Bf, _ = ctx.field(pos, mom, -1.0, LAM, 4, 64, 0.0)
+    gxf, gpf, _ = ctx.field_vjp(pos, mom, 2 * w[:, None] * Ef, 2 * w[:, None] * Bf, -1.0, LAM,
+                                4, 64, 0.0)
+    assert W.rel_l2(gxf, gx) <= 1e-12 and W.rel_l2(gpf, gp) <= 1e-12
+
+
 def test_fmm_vjp_eta0_equals_bruteforce_vjp(ctx):
     pos, mom = W.bunch(3000, seed=53)
     w = np.random.default_rng(54).random(len(pos))
@@ -71,7 +98,7 @@ def _same_topology(a, b):
     return all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
-@pytest.mark.parametrize("n", [3, 4])
+@pytest.mark.parametrize("n", [3, 4, 7, 9])
 def test_fmm_vjp_matches_finite_differences(ctx, n):
     N0, eta = 64, 0.5
     pos, mom = W.bunch(4096, seed=55)
@@ -157,7 +184,7 @@ def test_vjp_deterministic(ctx):
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
-@pytest.mark.parametrize("n", [2, 4, 6])
+@pytest.mark.parametrize("n", [2, 4, 6, 9])
 def test_mixed_vjp_close_to_fp64(ctx, n):
     """Mixed-precision reverse mode within 1e-5 (relative L2) of the FP64 one, for
     the loss L = sum w (|E|^2 + |B|^2) on a 2^15 bunch."""
