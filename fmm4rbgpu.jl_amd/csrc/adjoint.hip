@@ -543,9 +543,12 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
             T *A = ac[q];
             const T e0 = e[q][0], e1 = e[q][1], e2 = e[q][2];
             const T b0 = b[q][0], b1 = b[q][1], b2 = b[q][2];
-            T dxv[NP];
+            T dxv[NP], dx2v[NP];
 #pragma unroll
-            for (int si = 0; si < NP; ++si) dxv[si] = (T)(tx[q] - sn[si]);
+            for (int si = 0; si < NP; ++si) {
+                dxv[si] = (T)(tx[q] - sn[si]);
+                dx2v[si] = dxv[si] * dxv[si];
+            }
 #pragma unroll 1
             for (int sk = 0; sk < NP; ++sk) {
                 const T dz = (T)(tz[q] - sn[2 * NP + sk]);
@@ -557,7 +560,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                     const T v0 = fma(dy, b2, -(dz * b1));     // (R x b)_0
                     const T v1r = dz * b0, v2r = dy * b0;     // (R x b)_1 + dx b2, (R x b)_2 - dx b1
                     T G = 0, H = 0, Ax = 0, Ay = 0, Az = 0, Uy = 0, Uz = 0;
-                    T C0 = 0, C1 = 0, Cp = 0, Cpx = 0;
+                    T C0 = 0, C1 = 0, C2 = 0;  // sum c, sum c dx, sum c dx^2 over the row
 #pragma unroll
                     for (int si = 0; si < NP; ++si) {
                         const T dx = dxv[si];
@@ -581,12 +584,12 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                         const T v1 = fma(-dx, b2, v1r), v2 = fma(dx, b1, -v2r);
                         const T Ru = fma(m.x, fma(dx, e0, er), fma(m.y, v0, fma(m.z, v1, m.w * v2)));
                         const T c = y3 * (y2 * Ru);
-                        const T cp = c * pr;
                         C0 += c;
                         C1 = fma(c, dx, C1);
-                        Cp += cp;
-                        Cpx = fma(cp, dx, Cpx);
+                        C2 = fma(c, dx2v[si], C2);
                     }
+                    // pr = px dx + prjk along the row: sum c pr and sum c pr dx from C0..C2
+                    const T Cp = fma(px, C1, prjk * C0), Cpx = fma(px, C2, prjk * C1);
                     if (FWD) {
                         A[0] += H;
                         A[1] = fma(dy, G, A[1]);
@@ -732,7 +735,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
                     const T w0 = fma(m.z, dz, -(m.w * dy));  // (m x R)_0
                     const T w1r = -(m.y * dz), w2r = m.y * dy;
                     T Sxe = 0, Ex = 0, Ey = 0, Ez = 0, Bx = 0, By = 0, Bz = 0, Sxby = 0, Sxbz = 0;
-                    T C0 = 0, C1 = 0, Cp = 0;
+                    T C0 = 0, C1 = 0;
 #pragma unroll
                     for (int ti = 0; ti < NP; ++ti) {
                         const T dx = dxv[ti];
@@ -759,8 +762,8 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
                         const T c = y3 * (y2 * Ru);
                         C0 += c;
                         C1 = fma(c, dx, C1);
-                        Cp = fma(c, pr, Cp);
                     }
+                    const T Cp = fma(px, C1, prjk * C0);  // sum c pr (pr = px dx + prjk)
                     A[0] += fma(dy, Ey, fma(dz, Ez, Sxe));  // dl/dgamma = sum d^-3/2 e.R
                     A[1] += dy * Bz - dz * By;              // dl/dm = sum d^-3/2 R x b
                     A[2] += dz * Bx - Sxbz;

@@ -94,8 +94,8 @@ end
     field_vjp(positions, momenta, Ebar, Bbar, charge, alg::FMMHIP; lambda) -> (gpos, gmom)
 
 Reverse mode of `fields` at fixed tree and interaction lists (rebuilt, not stored):
-gpos = sum Ebar.dE/dpos + Bbar.dB/dpos, gmom likewise.  Computed in FP64 for either
-`alg.precision` (the mixed-precision fields agree with FP64 to 1e-5).
+gpos = sum Ebar.dE/dpos + Bbar.dB/dpos, gmom likewise, in `alg.precision` (the
+mixed-precision gradients agree with FP64 to ~2e-7 relative).
 """
 function field_vjp(positions::AbstractVector, momenta::AbstractVector, Ebar::AbstractVector,
                    Bbar::AbstractVector, charge::Real, alg::FMMHIP; lambda,
@@ -116,7 +116,8 @@ function field_vjp(positions::AbstractVector, momenta::AbstractVector, Ebar::Abs
                      Float64, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
                      Ptr{Cvoid}),
                     ctx.ptr, N, pointer(pos), pointer(mom), Float64(charge), Float64(lambda),
-                    Cint(alg.n), Int64(alg.N0), Float64(alg.eta), FP64, pointer(eb), pointer(bb),
+                    Cint(alg.n), Int64(alg.N0), Float64(alg.eta), precision_code(alg.precision),
+                    pointer(eb), pointer(bb),
                     pointer(gx), pointer(gp), C_NULL))
     end
     return gpos, gmom
