@@ -549,9 +549,14 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                 dxv[si] = (T)(tx[q] - sn[si]);
                 dx2v[si] = dxv[si] * dxv[si];
             }
+            // forward sums as in m2l_pair: H, U_y, U_z run over the source cluster;
+            // per plane G, A_x, A_y, A_z (also the totals of the adjoint's e G + b x A);
+            // per row G, A_x, A_z and the c sums (first node peeled: no resets)
+            T H = 0, Uy = 0, Uz = 0;
 #pragma unroll 1
             for (int sk = 0; sk < NP; ++sk) {
                 const T dz = (T)(tz[q] - sn[2 * NP + sk]);
+                T Gp = 0, Axp = 0, Ayp = 0, Azp = 0;
                 for (int sj = 0; sj < NP; ++sj) {
                     const T dy = (T)(ty[q] - sn[NP + sj]);
                     const T rrjk = fma(dy, dy, dz * dz) + (T)EPS64;
@@ -559,8 +564,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                     const T er = fma(dy, e1, dz * e2);        // e.R - dx e0
                     const T v0 = fma(dy, b2, -(dz * b1));     // (R x b)_0
                     const T v1r = dz * b0, v2r = dy * b0;     // (R x b)_1 + dx b2, (R x b)_2 - dx b1
-                    T G = 0, H = 0, Ax = 0, Ay = 0, Az = 0, Uy = 0, Uz = 0;
-                    T C0 = 0, C1 = 0, C2 = 0;  // sum c, sum c dx, sum c dx^2 over the row
+                    T G, Ax, Az, C0, C1, C2;  // row sums: c, c dx, c dx^2
 #pragma unroll
                     for (int si = 0; si < NP; ++si) {
                         const T dx = dxv[si];
@@ -571,37 +575,42 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                         else y = rsqrt_nr(d);
                         const T y2 = y * y, y3 = y2 * y;
                         const V4 m = smp[si + NP * (sj + NP * sk)];
-                        G = fma(y3, m.x, G);
-                        Ax = fma(y3, m.y, Ax);
-                        Ay = fma(y3, m.z, Ay);
-                        Az = fma(y3, m.w, Az);
-                        if (FWD) {  // forward row sums (see k_m2l)
+                        const T v1 = fma(-dx, b2, v1r), v2 = fma(dx, b1, -v2r);
+                        const T Ru = fma(m.x, fma(dx, e0, er), fma(m.y, v0, fma(m.z, v1, m.w * v2)));
+                        const T c = y3 * (y2 * Ru);
+                        if (si == 0) {
+                            G = y3 * m.x;
+                            Ax = y3 * m.y;
+                            Az = y3 * m.w;
+                            C0 = c;
+                            C1 = c * dx;
+                            C2 = c * dx2v[si];
+                        } else {
+                            G = fma(y3, m.x, G);
+                            Ax = fma(y3, m.y, Ax);
+                            Az = fma(y3, m.w, Az);
+                            C0 += c;
+                            C1 = fma(c, dx, C1);
+                            C2 = fma(c, dx2v[si], C2);
+                        }
+                        Ayp = fma(y3, m.z, Ayp);
+                        if (FWD) {
                             const T uu = y3 * dx;
                             H = fma(uu, m.x, H);
                             Uy = fma(uu, m.z, Uy);
                             Uz = fma(uu, m.w, Uz);
                         }
-                        const T v1 = fma(-dx, b2, v1r), v2 = fma(dx, b1, -v2r);
-                        const T Ru = fma(m.x, fma(dx, e0, er), fma(m.y, v0, fma(m.z, v1, m.w * v2)));
-                        const T c = y3 * (y2 * Ru);
-                        C0 += c;
-                        C1 = fma(c, dx, C1);
-                        C2 = fma(c, dx2v[si], C2);
                     }
                     // pr = px dx + prjk along the row: sum c pr and sum c pr dx from C0..C2
                     const T Cp = fma(px, C1, prjk * C0), Cpx = fma(px, C2, prjk * C1);
                     if (FWD) {
-                        A[0] += H;
-                        A[1] = fma(dy, G, A[1]);
-                        A[2] = fma(dz, G, A[2]);
-                        A[3] += dz * Ay - dy * Az;
-                        A[4] += Uz - dz * Ax;
-                        A[5] += dy * Ax - Uy;
+                        A[1] = fma(dy, G, A[1]);   // E_y
+                        A[3] = fma(-dy, Az, A[3]);  // B_x (dy part)
+                        A[5] = fma(dy, Ax, A[5]);   // B_z (dy part)
                     }
-                    A[6] += G;
-                    A[7] += Ax;
-                    A[8] += Ay;
-                    A[9] += Az;
+                    Gp += G;
+                    Axp += Ax;
+                    Azp += Az;
                     A[10] += C1;
                     A[11] = fma(dy, C0, A[11]);
                     A[12] = fma(dz, C0, A[12]);
@@ -610,6 +619,20 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                     A[15] = fma(dy, Cp, A[15]);
                     A[16] = fma(dz, Cp, A[16]);
                 }
+                if (FWD) {
+                    A[2] = fma(dz, Gp, A[2]);   // E_z
+                    A[3] = fma(dz, Ayp, A[3]);  // B_x (dz part)
+                    A[4] = fma(-dz, Axp, A[4]);  // B_y (dz part)
+                }
+                A[6] += Gp;
+                A[7] += Axp;
+                A[8] += Ayp;
+                A[9] += Azp;
+            }
+            if (FWD) {
+                A[0] += H;
+                A[4] += Uz;
+                A[5] -= Uy;
             }
             if constexpr (MIXED)
                 for (int k = 0; k < NA; ++k) ad[q][k] += (double)A[k], A[k] = 0;

@@ -37,56 +37,74 @@ __device__ inline double rsq<double>(double d) { return rsqrt_nr(d); }
 template <>
 __device__ inline float rsq<float>(float d) { return __builtin_amdgcn_rsqf(d); }
 
+// One source cluster's contribution to one target node.  Per node pair: the
+// geometry (3 operations on row-hoisted terms), d^-1/2, y3 and 8 products/sums.
+// Running sums over the whole source cluster: H (E_x), U_y, U_z; per plane (dz
+// fixed): G, A_x, A_y; per row (dy fixed): G, A_x, A_z.  So E = (H, sum_rows dy G,
+// sum_planes dz G), B = (sum dz A_y - sum dy A_z, U_z - sum dz A_x, sum dy A_x - U_y).
+// The first node of every row is peeled so that no row sum needs a reset.
 template <int NP, typename T, typename V4>
 __device__ inline void m2l_pair(double tx, double ty, double tz, const double *sn, const V4 *smp,
                                 T px, T py, T pz, double *acc) {
-    T dxr[NP], dx2[NP];
+    T dxr[NP], dx2[NP], dyr[NP], dy2[NP];
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
         dxr[i] = (T)(tx - sn[i]);
         dx2[i] = dxr[i] * dxr[i];
+        dyr[i] = (T)(ty - sn[NP + i]);
+        dy2[i] = fma(dyr[i], dyr[i], (T)EPS64);
     }
-    T e0 = 0, e1 = 0, e2 = 0, b0 = 0, b1 = 0, b2 = 0;
+    T H = 0, Uy = 0, Uz = 0, e1 = 0, e2 = 0, b0 = 0, b1 = 0, b2 = 0;
     for (int sk = 0; sk < NP; ++sk) {
         const T dz = (T)(tz - sn[2 * NP + sk]);
+        const T dz2 = dz * dz, pzdz = pz * dz;
+        T Gp = 0, Axp = 0, Ayp = 0;
+#pragma unroll
         for (int sj = 0; sj < NP; ++sj) {
-            const T dy = (T)(ty - sn[NP + sj]);
-            const T rrjk = fma(dy, dy, dz * dz) + (T)EPS64;
-            const T prjk = fma(py, dy, pz * dz);
-            T G = 0, H = 0, Ax = 0, Ay = 0, Az = 0, Uy = 0, Uz = 0;
+            const T dy = dyr[sj];
+            const T rrjk = dy2[sj] + dz2;
+            const T prjk = fma(py, dy, pzdz);
             const V4 *row = smp + NP * (sj + NP * sk);
+            T G, Ax, Az;
 #pragma unroll
             for (int si = 0; si < NP; ++si) {
                 const T dx = dxr[si];
                 const T rr = dx2[si] + rrjk;
                 const T pr = fma(px, dx, prjk);
-                const T d = fma(pr, pr, rr);
-                const T y = rsq<T>(d);
+                const T y = rsq<T>(fma(pr, pr, rr));
                 const T y3 = (y * y) * y;
                 const V4 m = row[si];
                 const T u = y3 * dx;
-                G = fma(y3, m.x, G);
+                if (si == 0) {
+                    G = y3 * m.x;
+                    Ax = y3 * m.y;
+                    Az = y3 * m.w;
+                } else {
+                    G = fma(y3, m.x, G);
+                    Ax = fma(y3, m.y, Ax);
+                    Az = fma(y3, m.w, Az);
+                }
+                Ayp = fma(y3, m.z, Ayp);
                 H = fma(u, m.x, H);
-                Ax = fma(y3, m.y, Ax);
-                Ay = fma(y3, m.z, Ay);
-                Az = fma(y3, m.w, Az);
                 Uy = fma(u, m.z, Uy);
                 Uz = fma(u, m.w, Uz);
             }
-            e0 += H;
             e1 = fma(dy, G, e1);
-            e2 = fma(dz, G, e2);
-            b0 += dz * Ay - dy * Az;
-            b1 += Uz - dz * Ax;
-            b2 += dy * Ax - Uy;
+            b0 = fma(-dy, Az, b0);
+            b2 = fma(dy, Ax, b2);
+            Gp += G;
+            Axp += Ax;
         }
+        e2 = fma(dz, Gp, e2);
+        b0 = fma(dz, Ayp, b0);
+        b1 = fma(-dz, Axp, b1);
     }
-    acc[0] += (double)e0;
+    acc[0] += (double)H;
     acc[1] += (double)e1;
     acc[2] += (double)e2;
     acc[3] += (double)b0;
-    acc[4] += (double)b1;
-    acc[5] += (double)b2;
+    acc[4] += (double)(b1 + Uz);
+    acc[5] += (double)(b2 - Uy);
 }
 
 template <int NP, bool MIXED>
