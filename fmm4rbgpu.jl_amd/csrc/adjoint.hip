@@ -879,9 +879,10 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ le
                                                   double *__restrict__ gx) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using V4 = typename std::conditional<MIXED, float4, double4>::type;
-    __shared__ double4 sx[PT];
-    __shared__ V4 sq[PT];
+    extern __shared__ double4 adj_lds[];  // sized by the launch: blockDim.x entries each
     const int BT = blockDim.x, tid = threadIdx.x;
+    double4 *sx = adj_lds;
+    V4 *sq = reinterpret_cast<V4 *>(adj_lds + BT);
     const int lane = tid & 63, li = lane & 15, g = lane >> 4, wbase = tid - lane;
     const int32_t t = leaves[xcd_item(blockIdx.x, gridDim.x)];
     const int32_t a = lo[t], ct = cnt[t];
@@ -988,9 +989,10 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
     constexpr int NA = 17;
     // FP64 keeps 17 FP64 sums per source: 2 sources per lane (32 lanes x 2 groups)
     constexpr int NG = MIXED ? PNG : 2, TL = 64 / NG;
-    __shared__ double4 tx[PT];
-    __shared__ V4 te[PT], tb4[PT];
+    extern __shared__ double4 adj_lds[];  // sized by the launch: blockDim.x entries each
     const int BT = blockDim.x, tid = threadIdx.x;
+    double4 *tx = adj_lds;
+    V4 *te = reinterpret_cast<V4 *>(adj_lds + BT), *tb4 = te + BT;
     const int lane = tid & 63, li = lane % TL, g = lane / TL, wbase = tid - lane;
     const int32_t s = leaves[xcd_item(blockIdx.x, gridDim.x)];
     const int32_t a = lo[s], cs = cnt[s];
@@ -1355,18 +1357,21 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     // 7. P2P^T
     int bt = ((S.maxleaf + 63) / 64) * 64;
     bt = std::max(64, std::min(bt, PT));
+    const size_t v4 = mixed ? sizeof(float4) : sizeof(double4);
+    const size_t lds_t = (size_t)bt * (sizeof(double4) + v4);
+    const size_t lds_s = (size_t)bt * (sizeof(double4) + 2 * v4);
     if (mixed) {
-        hipLaunchKernelGGL(k_adj_p2p_t<true>, dim3(nleaf), dim3(bt), 0, st, leaves,
+        hipLaunchKernelGGL(k_adj_p2p_t<true>, dim3(nleaf), dim3(bt), lds_t, st, leaves,
                            c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, lo, cnt, sp8, ob, gx);
         FMM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_adj_p2p_s<true>, dim3(nleaf), dim3(bt), 0, st, leaves,
+        hipLaunchKernelGGL(k_adj_p2p_s<true>, dim3(nleaf), dim3(bt), lds_s, st, leaves,
                            c.d_p2pT_off.as<int32_t>(), c.p2pT_sorted, c.sb, lo, cnt, sp8, ob, gx,
                            gp);
     } else {
-        hipLaunchKernelGGL(k_adj_p2p_t<false>, dim3(nleaf), dim3(bt), 0, st, leaves,
+        hipLaunchKernelGGL(k_adj_p2p_t<false>, dim3(nleaf), dim3(bt), lds_t, st, leaves,
                            c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, lo, cnt, sp8, ob, gx);
         FMM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_adj_p2p_s<false>, dim3(nleaf), dim3(bt), 0, st, leaves,
+        hipLaunchKernelGGL(k_adj_p2p_s<false>, dim3(nleaf), dim3(bt), lds_s, st, leaves,
                            c.d_p2pT_off.as<int32_t>(), c.p2pT_sorted, c.sb, lo, cnt, sp8, ob, gx,
                            gp);
     }

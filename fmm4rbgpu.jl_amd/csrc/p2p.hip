@@ -7,12 +7,14 @@
 //
 // One workgroup owns one target leaf, walks the leaf's CSR source list and
 // stages each source leaf in LDS (x, y, z in FP64; gamma, px, py, pz).  A wave
-// covers 64 targets as (64 / NG) target lanes x NG source groups: lane (ti, g)
-// holds the NG targets ti + (64 / NG) q and takes the staged sources
-// j = g (mod NG), so one LDS read serves NG pairs (with NG = 1, a broadcast per
-// target moves 40 bytes of LDS data per pair and the kernel is bound by LDS
-// bandwidth).  The groups' partial sums are combined with xor-shuffles at the
-// end, in a fixed order.
+// covers 16 TPL targets as 16 target lanes x 4 source groups: lane (ti, g) holds
+// the TPL targets ti + 16 q and takes the staged sources j = g (mod 4), so one
+// LDS read serves TPL pairs (one target per lane moves 40 bytes of LDS data per
+// pair and is bound by LDS bandwidth).  Each staged tile's partial sums are added
+// into the lane's FP64 sums; the 4 groups' sums meet in two xor-shuffles at the
+// end, in a fixed order.  TPL = 4 (64 targets per wave), or 8 for FP64 with
+// leaves above 64 particles (reducing per tile instead, to keep fewer FP64 sums
+// per lane, measured 1.3x slower).
 // Results go to a tree-order near-field array that L2P adds to the far field.
 // Mixed precision: R = x_i - x_j is formed in FP64 and rounded once to FP32
 // (the closest pairs dominate the field of their particles, so R must keep
@@ -23,16 +25,15 @@
 
 namespace fmm4rb {
 
-constexpr int P2P_MAXT = 256;  // threads per block: up to 4 waves, 256 targets per pass
-constexpr int P2P_NG = 4;      // source groups per wave (1 and 2 measured slower at N = 2^22)
+constexpr int P2P_MAXT = 512;  // threads per block: up to 8 waves
 
 template <typename T>
 __device__ inline void p2p_pair(const double *xi, double sx, double sy, double sz, T g, T px,
                                 T py, T pz, T *a) {
     const T R0 = (T)(xi[0] - sx), R1 = (T)(xi[1] - sy), R2 = (T)(xi[2] - sz);
-    const T rr = fma(R2, R2, fma(R1, R1, R0 * R0));
+    const T rr = fma(R2, R2, fma(R1, R1, fma(R0, R0, (T)EPS64)));
     const T pr = fma(pz, R2, fma(py, R1, px * R0));
-    const T d = fma(pr, pr, rr) + (T)EPS64;
+    const T d = fma(pr, pr, rr);
     T y;
     if constexpr (sizeof(T) == 4) y = __builtin_amdgcn_rsqf(d);
     else y = rsqrt_nr(d);
@@ -49,7 +50,7 @@ __device__ inline void p2p_pair(const double *xi, double sx, double sy, double s
     a[5] = fma(y3, c2, a[5]);
 }
 
-template <bool MIXED, int NG>
+template <bool MIXED, int TPL>
 __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
     const int32_t *__restrict__ leaves, const int32_t *__restrict__ off,
     const uint64_t *__restrict__ keys, int sb, const int32_t *__restrict__ lo,
@@ -57,25 +58,28 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
     double *__restrict__ nearf) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using V4 = typename std::conditional<MIXED, float4, double4>::type;
-    __shared__ double4 tx[P2P_MAXT];  // source x, y, z (FP64)
-    __shared__ V4 tq[P2P_MAXT];       // source gamma, px, py, pz
+    // LDS sized by the launch (blockDim.x entries each), not by P2P_MAXT: a one-wave
+    // block then holds 3 KB (mixed) and LDS does not limit the waves per CU
+    extern __shared__ double4 p2p_lds[];
     const int BT = blockDim.x;
+    double4 *tx = p2p_lds;                             // source x, y, z (FP64)
+    V4 *tq = reinterpret_cast<V4 *>(p2p_lds + BT);     // source gamma, px, py, pz
     const int tid = threadIdx.x;
-    constexpr int TL = 64 / NG;  // target lanes per wave
-    const int lane = tid & 63, ti = lane % TL, g = lane / TL;
-    const int wbase = tid - lane;  // first target of this wave within a pass
+    const int lane = tid & 63, ti = lane & 15, g = lane >> 4;
+    const int tw = (tid >> 6) * 16 * TPL;  // first target of this wave within a pass
+    const int per_pass = (BT >> 6) * 16 * TPL;
     const int32_t t = leaves[xcd_item(blockIdx.x, gridDim.x)];
     const int32_t a = lo[t], ct = cnt[t];
     const int32_t beg = off[t], end = off[t + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
     const double4 *p8 = reinterpret_cast<const double4 *>(sp8);
     const float4 *q4 = reinterpret_cast<const float4 *>(sp4f);
-    for (int tb = 0; tb < ct; tb += BT) {
-        double xi[NG][3];
-        double acc[NG][6];
+    for (int tb = 0; tb < ct; tb += per_pass) {
+        double xi[TPL][3];
+        double acc[TPL][6];
 #pragma unroll
-        for (int q = 0; q < NG; ++q) {
-            const int i = min(tb + wbase + ti + TL * q, ct - 1);
+        for (int q = 0; q < TPL; ++q) {
+            const int i = min(tb + tw + ti + 16 * q, ct - 1);
             const double4 x4 = p8[2 * (int64_t)(a + i)];
             xi[q][0] = x4.x;
             xi[q][1] = x4.y;
@@ -101,63 +105,86 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
                 }
                 __syncthreads();
                 const int m = min(BT, cs - s0);
-                T part[NG][6];
+                if constexpr (MIXED) {
+                    T part[TPL][6];
 #pragma unroll
-                for (int q = 0; q < NG; ++q)
+                    for (int q = 0; q < TPL; ++q)
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) part[q][k] = 0;
-                for (int jj = g; jj < m; jj += NG) {
-                    const double4 xs = tx[jj];
-                    const V4 sq = tq[jj];  // gamma, px, py, pz
+                        for (int k = 0; k < 6; ++k) part[q][k] = 0;
+                    for (int jj = g; jj < m; jj += 4) {
+                        const double4 xs = tx[jj];
+                        const V4 sq = tq[jj];  // gamma, px, py, pz
 #pragma unroll
-                    for (int q = 0; q < NG; ++q)
-                        p2p_pair<T>(xi[q], xs.x, xs.y, xs.z, sq.x, sq.y, sq.z, sq.w, part[q]);
+                        for (int q = 0; q < TPL; ++q)
+                            p2p_pair<T>(xi[q], xs.x, xs.y, xs.z, sq.x, sq.y, sq.z, sq.w, part[q]);
+                    }
+#pragma unroll
+                    for (int q = 0; q < TPL; ++q)
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) acc[q][k] += (double)part[q][k];
+                } else {
+                    for (int jj = g; jj < m; jj += 4) {
+                        const double4 xs = tx[jj];
+                        const V4 sq = tq[jj];
+#pragma unroll
+                        for (int q = 0; q < TPL; ++q)
+                            p2p_pair<T>(xi[q], xs.x, xs.y, xs.z, sq.x, sq.y, sq.z, sq.w, acc[q]);
+                    }
                 }
-#pragma unroll
-                for (int q = 0; q < NG; ++q)
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) acc[q][k] += (double)part[q][k];
             }
         }
-        // Sum the source groups (lanes ti + TL g), fixed order.
+        // Sum the 4 source groups (lanes ti + 16 g), fixed order.
 #pragma unroll
-        for (int q = 0; q < NG; ++q)
+        for (int q = 0; q < TPL; ++q)
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
-                if (NG >= 4) acc[q][k] += __shfl_xor(acc[q][k], 16);
-                if (NG >= 2) acc[q][k] += __shfl_xor(acc[q][k], 32);
+                acc[q][k] += __shfl_xor(acc[q][k], 16);
+                acc[q][k] += __shfl_xor(acc[q][k], 32);
             }
-        // Lane (ti, g) writes its target ti + TL g.
-        double out[6];
+        // lane (ti, g) writes targets ti + 16 (g + 4 r)
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            out[k] = acc[0][k];
+        for (int r = 0; r < TPL / 4; ++r) {
+            double out[6];
 #pragma unroll
-            for (int q = 1; q < NG; ++q)
-                if (g == q) out[k] = acc[q][k];
-        }
-        const int i = tb + wbase + ti + TL * g;
-        if (i < ct) {
-            double *o = nearf + 6 * (int64_t)(a + i);
+            for (int k = 0; k < 6; ++k) {
+                out[k] = acc[4 * r][k];
 #pragma unroll
-            for (int k = 0; k < 6; ++k) o[k] = out[k];
+                for (int u = 1; u < 4; ++u)
+                    if (g == u) out[k] = acc[4 * r + u][k];
+            }
+            const int i = tb + tw + ti + 16 * (g + 4 * r);
+            if (i < ct) {
+                double *o = nearf + 6 * (int64_t)(a + i);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) o[k] = out[k];
+            }
         }
     }
 }
 
 void launch_p2p(Ctx &c, const int32_t *off, const uint64_t *keys, int sb) {
     const unsigned nleaf = (unsigned)c.shape.leaves.size();
-    // one wave per 64 targets of the largest leaf, at most P2P_MAXT threads
-    int bt = ((c.shape.maxleaf + 63) / 64) * 64;
-    bt = std::max(64, std::min(bt, P2P_MAXT));
     const bool mixed = c.precision == FMM4RB_MIXED;
     const float *q4 = mixed ? c.d_sp8f.as<float>() : (const float *)nullptr;
-#define P2P_L(MX)                                                                                \
-    hipLaunchKernelGGL((k_p2p<MX, P2P_NG>), dim3(nleaf), dim3(bt), 0, c.stream,                  \
+    // targets per lane: 4 (a wave per 64 targets); FP64 with leaves above 64
+    // particles 8 (a wave per 128 targets).  Measured at 25.6M particles (~98 per
+    // leaf): FP64 P2P 134 ms with 8 vs 144 ms with 4; mixed 99 ms with 8 (256 VGPRs)
+    // vs 92 ms with 4.
+    const int tpl = (!mixed && c.shape.maxleaf > 64) ? 8 : 4;
+    int bt = 64 * ((c.shape.maxleaf + 16 * tpl - 1) / (16 * tpl));
+    bt = std::max(64, std::min(bt, P2P_MAXT));
+    const size_t lds = (size_t)bt * (sizeof(double4) + (mixed ? sizeof(float4) : sizeof(double4)));
+#define P2P_L(MX, TP)                                                                            \
+    hipLaunchKernelGGL((k_p2p<MX, TP>), dim3(nleaf), dim3(bt), lds, c.stream,                    \
                        c.d_leaves.as<int32_t>(), off, keys, sb, c.d_lo.as<int32_t>(),             \
                        c.d_cnt.as<int32_t>(), c.d_sp8.as<double>(), q4, c.d_near.as<double>())
-    if (mixed) P2P_L(true);
-    else P2P_L(false);
+    if (mixed) {
+        if (tpl == 4) P2P_L(true, 4);
+        else P2P_L(true, 8);
+    } else {
+        if (tpl == 4) P2P_L(false, 4);
+        else P2P_L(false, 8);
+    }
 #undef P2P_L
     FMM_LAUNCH_CHECK();
 }
