@@ -19,8 +19,8 @@
 // Mixed precision: R = x_i - x_j is formed in FP64 and rounded once to FP32
 // (the closest pairs dominate the field of their particles, so R must keep
 // its relative accuracy however small it is); the kernel then runs in FP32
-// with v_rsq_f32, and each staged tile's FP32 partial sums are added into FP64
-// accumulators.
+// with v_rsq_f32, and each staged tile's FP32 partial sums are added into
+// Kahan-compensated FP32 accumulators, combined in FP64 at the end.
 #include "fmm4rb_internal.hpp"
 
 namespace fmm4rb {
@@ -76,7 +76,10 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
     const float4 *q4 = reinterpret_cast<const float4 *>(sp4f);
     for (int tb = 0; tb < ct; tb += per_pass) {
         double xi[TPL][3];
-        double acc[TPL][6];
+        // FP64: the sums.  MIXED: Kahan-compensated FP32 sums of the tiles' partial
+        // sums (hi, cmp), half the registers of FP64 sums
+        double acc[MIXED ? 1 : TPL][6];
+        float hi[MIXED ? TPL : 1][6], cmp[MIXED ? TPL : 1][6];
 #pragma unroll
         for (int q = 0; q < TPL; ++q) {
             const int i = min(tb + tw + ti + 16 * q, ct - 1);
@@ -85,7 +88,10 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
             xi[q][1] = x4.y;
             xi[q][2] = x4.z;
 #pragma unroll
-            for (int m = 0; m < 6; ++m) acc[q][m] = 0.0;
+            for (int m = 0; m < 6; ++m) {
+                if constexpr (MIXED) hi[q][m] = cmp[q][m] = 0.0f;
+                else acc[q][m] = 0.0;
+            }
         }
         for (int32_t e = beg; e < end; ++e) {
             const int32_t s = (int32_t)(keys[e] & mask);
@@ -121,7 +127,12 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
 #pragma unroll
                     for (int q = 0; q < TPL; ++q)
 #pragma unroll
-                        for (int k = 0; k < 6; ++k) acc[q][k] += (double)part[q][k];
+                        for (int k = 0; k < 6; ++k) {  // Kahan: hi - cmp is the running sum
+                            const float y = part[q][k] - cmp[q][k];
+                            const float t = hi[q][k] + y;
+                            cmp[q][k] = (t - hi[q][k]) - y;
+                            hi[q][k] = t;
+                        }
                 } else {
                     for (int jj = g; jj < m; jj += 4) {
                         const double4 xs = tx[jj];
@@ -133,13 +144,18 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
                 }
             }
         }
-        // Sum the 4 source groups (lanes ti + 16 g), fixed order.
+        // Sum the 4 source groups (lanes ti + 16 g), fixed order, in FP64.
+        double tot[TPL][6];
 #pragma unroll
         for (int q = 0; q < TPL; ++q)
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
-                acc[q][k] += __shfl_xor(acc[q][k], 16);
-                acc[q][k] += __shfl_xor(acc[q][k], 32);
+                double v;
+                if constexpr (MIXED) v = (double)hi[q][k] - (double)cmp[q][k];
+                else v = acc[q][k];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                tot[q][k] = v;
             }
         // lane (ti, g) writes targets ti + 16 (g + 4 r)
 #pragma unroll
@@ -147,10 +163,10 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
             double out[6];
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
-                out[k] = acc[4 * r][k];
+                out[k] = tot[4 * r][k];
 #pragma unroll
                 for (int u = 1; u < 4; ++u)
-                    if (g == u) out[k] = acc[4 * r + u][k];
+                    if (g == u) out[k] = tot[4 * r + u][k];
             }
             const int i = tb + tw + ti + 16 * (g + 4 * r);
             if (i < ct) {
