@@ -60,10 +60,11 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
     using V4 = typename std::conditional<MIXED, float4, double4>::type;
     // LDS sized by the launch (blockDim.x entries each), not by P2P_MAXT: a one-wave
     // block then holds 3 KB (mixed) and LDS does not limit the waves per CU
-    extern __shared__ double4 p2p_lds[];
+    extern __shared__ double2 p2p_lds[];
     const int BT = blockDim.x;
-    double4 *tx = p2p_lds;                             // source x, y, z (FP64)
-    V4 *tq = reinterpret_cast<V4 *>(p2p_lds + BT);     // source gamma, px, py, pz
+    double2 *txy = p2p_lds;                                        // source x, y (FP64)
+    double *tz = reinterpret_cast<double *>(p2p_lds + BT);         // source z
+    V4 *tq = reinterpret_cast<V4 *>(p2p_lds + BT + (BT + 1) / 2);  // gamma, px, py, pz
     const int tid = threadIdx.x;
     const int lane = tid & 63, ti = lane & 15, g = lane >> 4;
     const int tw = (tid >> 6) * 16 * TPL;  // first target of this wave within a pass
@@ -101,7 +102,8 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
                 const int j = s0 + tid;
                 if (j < cs) {
                     const double4 x4 = p8[2 * (int64_t)(sa + j)];
-                    tx[tid] = x4;
+                    txy[tid] = make_double2(x4.x, x4.y);
+                    tz[tid] = x4.z;
                     if constexpr (MIXED) {
                         tq[tid] = q4[sa + j];
                     } else {
@@ -118,11 +120,12 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
 #pragma unroll
                         for (int k = 0; k < 6; ++k) part[q][k] = 0;
                     for (int jj = g; jj < m; jj += 4) {
-                        const double4 xs = tx[jj];
+                        const double2 xy = txy[jj];
+                        const double zs = tz[jj];
                         const V4 sq = tq[jj];  // gamma, px, py, pz
 #pragma unroll
                         for (int q = 0; q < TPL; ++q)
-                            p2p_pair<T>(xi[q], xs.x, xs.y, xs.z, sq.x, sq.y, sq.z, sq.w, part[q]);
+                            p2p_pair<T>(xi[q], xy.x, xy.y, zs, sq.x, sq.y, sq.z, sq.w, part[q]);
                     }
 #pragma unroll
                     for (int q = 0; q < TPL; ++q)
@@ -135,11 +138,12 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
                         }
                 } else {
                     for (int jj = g; jj < m; jj += 4) {
-                        const double4 xs = tx[jj];
+                        const double2 xy = txy[jj];
+                        const double zs = tz[jj];
                         const V4 sq = tq[jj];
 #pragma unroll
                         for (int q = 0; q < TPL; ++q)
-                            p2p_pair<T>(xi[q], xs.x, xs.y, xs.z, sq.x, sq.y, sq.z, sq.w, acc[q]);
+                            p2p_pair<T>(xi[q], xy.x, xy.y, zs, sq.x, sq.y, sq.z, sq.w, acc[q]);
                     }
                 }
             }
@@ -189,7 +193,10 @@ void launch_p2p(Ctx &c, const int32_t *off, const uint64_t *keys, int sb) {
     const int tpl = (!mixed && c.shape.maxleaf > 64) ? 8 : 4;
     int bt = 64 * ((c.shape.maxleaf + 16 * tpl - 1) / (16 * tpl));
     bt = std::max(64, std::min(bt, P2P_MAXT));
-    const size_t lds = (size_t)bt * (sizeof(double4) + (mixed ? sizeof(float4) : sizeof(double4)));
+    // per source: x, y (16 B), z (8 B), gamma and p (16 B mixed, 32 B FP64): 40 B read per
+    // source and lane in mixed precision instead of 48 with a padded double4
+    const size_t lds = (size_t)bt * sizeof(double2) + (size_t)((bt + 1) / 2) * sizeof(double2) +
+                       (size_t)bt * (mixed ? sizeof(float4) : sizeof(double4));
 #define P2P_L(MX, TP)                                                                            \
     hipLaunchKernelGGL((k_p2p<MX, TP>), dim3(nleaf), dim3(bt), lds, c.stream,                    \
                        c.d_leaves.as<int32_t>(), off, keys, sb, c.d_lo.as<int32_t>(),             \
