@@ -88,6 +88,20 @@ def test_fmm_vjp_eta0_equals_bruteforce_vjp(ctx):
     assert W.rel_l2(gx, gxb) <= 1e-12 and W.rel_l2(gp, gpb) <= 1e-12
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_fmm_vjp_eta0_large_leaves(ctx, prec):
+    """N0 = 1024 (leaves of up to 1024 particles: several passes per block) with
+    eta = 0: the FMM reverse mode is P2P^T only and equals the BruteForce VJP."""
+    pos, mom = W.bunch(5000, seed=72)
+    w = np.random.default_rng(73).random(len(pos))
+    E, B = ctx.bruteforce(pos, mom, -1.0, LAM)
+    Eb, Bb = 2 * w[:, None] * E, 2 * w[:, None] * B
+    gxb, gpb = ctx.bruteforce_vjp(pos, mom, Eb, Bb, -1.0, LAM)
+    gx, gp, _ = ctx.field_vjp(pos, mom, Eb, Bb, -1.0, LAM, 4, 1024, 0.0, precision=prec)
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    assert W.rel_l2(gx, gxb) <= tol and W.rel_l2(gp, gpb) <= tol
+
+
 def _topology(ctx, pos, mom, N0, eta):
     t = ctx.tree(pos, mom, N0)
     p2p, m2l = ctx.lists(pos, mom, N0, eta)

@@ -34,10 +34,15 @@ def _case(name):
         return (np.array([[0.1, 0.2, 0.3]]), np.array([[0.0, 0.0, 1.0]])), 125
     if name == "small_n0":
         return W.bunch(1000, seed=5), 7
+    if name == "n0_1":  # one particle per leaf
+        return W.bunch(300, seed=6), 1
+    if name == "n0_1024":  # leaves of up to 1024 particles: several target passes per block
+        return W.bunch(20000, seed=7), 1024
     raise KeyError(name)
 
 
-CASES = ["uniform8192", "bunch32k", "ties", "earlyleaf", "single", "small_n0"]
+CASES = ["uniform8192", "bunch32k", "ties", "earlyleaf", "single", "small_n0", "n0_1",
+         "n0_1024"]
 
 
 def _oracle_tree(orc, pos, N0, pavg):
@@ -124,12 +129,15 @@ def test_fp64_fmm_uniform8192(ctx, orc, n):
     assert np.array_equal(B, np.zeros_like(B)) and np.array_equal(Bo, np.zeros_like(Bo))
 
 
-@pytest.mark.parametrize("name", ["bunch32k", "earlyleaf", "single", "small_n0", "ties"])
+@pytest.mark.parametrize("name", ["bunch32k", "earlyleaf", "single", "small_n0", "ties", "n0_1",
+                                  "n0_1024"])
 def test_fp64_fmm_cases(ctx, orc, name):
     (pos, mom), N0 = _case(name)
     E, B, st = ctx.field(pos, mom, -1.0, 1e-3, 4, N0, 0.5, "fp64")
     Eo, Bo = orc.fmm(pos, mom, -1.0, 1e-3, 4, N0, 0.5, pavg=np.array(st.p_avg))
     assert W.rel_l2(E, Eo) <= FP64_TOL and W.rel_l2(B, Bo) <= FP64_TOL
+    Em, Bm, _ = ctx.field(pos, mom, -1.0, 1e-3, 4, N0, 0.5, "mixed")
+    assert W.rel_l2(Em, E) <= MIXED_TOL and W.rel_l2(Bm, B) <= MIXED_TOL
 
 
 @pytest.mark.parametrize("n", [2, 4, 6, 7])
