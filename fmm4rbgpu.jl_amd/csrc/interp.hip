@@ -149,14 +149,27 @@ __global__ __launch_bounds__(OP_T) void k_p2m(const int32_t *__restrict__ leaves
 }
 
 // ------------------------------------------------------- M2M (gather)
+// Components transferred together per barrier phase: the largest divisor of NC
+// whose three LDS buffers of NN doubles fit in 48 KB.
+template <int NP, int NC>
+struct TransferGroup {
+    static constexpr int NN = NP * NP * NP;
+    static constexpr int FIT = (48 * 1024) / (3 * NN * 8);
+    static constexpr int CG = FIT >= NC ? NC
+                              : (NC % 3 == 0 && FIT >= 3) ? 3
+                              : (NC % 2 == 0 && FIT >= 2) ? 2
+                                                          : 1;
+};
+
 template <int NP>
 __global__ __launch_bounds__(OP_T) void k_m2m(const int32_t *__restrict__ parents,
                                               const double *__restrict__ ncoord,
                                               double *__restrict__ mp4) {
     constexpr int NN = NP * NP * NP, TPT = (NN + OP_T - 1) / OP_T;
+    constexpr int CG = TransferGroup<NP, 4>::CG;
     __shared__ double Wm[3][NP][NP];  // [axis][child node][parent node]
     __shared__ double pn[3 * NP], cn[3 * NP];
-    __shared__ double bin[NN], ba[NN], bb[NN];
+    __shared__ double bin[CG][NN], ba[CG][NN], bb[CG][NN];
     const int tid = threadIdx.x;
     const int32_t P = parents[blockIdx.x];
     if (tid < 3 * NP) pn[tid] = ncoord[(int64_t)P * 3 * NP + tid];
@@ -174,23 +187,26 @@ __global__ __launch_bounds__(OP_T) void k_m2m(const int32_t *__restrict__ parent
             const int d = tid / NP, ci = tid % NP;
             lgw<NP>(cn[tid], pn + d * NP, &Wm[d][ci][0]);
         }
-#pragma unroll
-        for (int comp = 0; comp < 4; ++comp) {
+        for (int c0 = 0; c0 < 4; c0 += CG) {
             __syncthreads();
-            for (int e = tid; e < NN; e += OP_T) bin[e] = mp4[((int64_t)C * NN + e) * 4 + comp];
-            __syncthreads();
-            for (int e = tid; e < NN; e += OP_T) {  // contract x: ba[Pi, cj, ck]
-                const int Pi = e % NP, r = e / NP;
-                double s = 0.0;
-                for (int ci = 0; ci < NP; ++ci) s += Wm[0][ci][Pi] * bin[ci + NP * r];
-                ba[e] = s;
+            for (int e = tid; e < CG * NN; e += OP_T) {  // contiguous when CG = 4
+                const int n = e / CG, cc = e % CG;
+                bin[cc][n] = mp4[((int64_t)C * NN + n) * 4 + c0 + cc];
             }
             __syncthreads();
-            for (int e = tid; e < NN; e += OP_T) {  // contract y: bb[Pi, Pj, ck]
-                const int Pi = e % NP, Pj = (e / NP) % NP, ck = e / (NP * NP);
+            for (int e = tid; e < CG * NN; e += OP_T) {  // contract x: ba[Pi, cj, ck]
+                const int cc = e / NN, n = e % NN, Pi = n % NP, r = n / NP;
                 double s = 0.0;
-                for (int cj = 0; cj < NP; ++cj) s += Wm[1][cj][Pj] * ba[Pi + NP * (cj + NP * ck)];
-                bb[e] = s;
+                for (int ci = 0; ci < NP; ++ci) s += Wm[0][ci][Pi] * bin[cc][ci + NP * r];
+                ba[cc][n] = s;
+            }
+            __syncthreads();
+            for (int e = tid; e < CG * NN; e += OP_T) {  // contract y: bb[Pi, Pj, ck]
+                const int cc = e / NN, n = e % NN;
+                const int Pi = n % NP, Pj = (n / NP) % NP, ck = n / (NP * NP);
+                double s = 0.0;
+                for (int cj = 0; cj < NP; ++cj) s += Wm[1][cj][Pj] * ba[cc][Pi + NP * (cj + NP * ck)];
+                bb[cc][n] = s;
             }
             __syncthreads();
 #pragma unroll
@@ -198,9 +214,12 @@ __global__ __launch_bounds__(OP_T) void k_m2m(const int32_t *__restrict__ parent
                 const int e = tid + q * OP_T;
                 if (e < NN) {
                     const int ij = e % (NP * NP), Pk = e / (NP * NP);
-                    double s = 0.0;
-                    for (int ck = 0; ck < NP; ++ck) s += Wm[2][ck][Pk] * bb[ij + NP * NP * ck];
-                    acc[q][comp] += s;
+#pragma unroll
+                    for (int cc = 0; cc < CG; ++cc) {
+                        double s = 0.0;
+                        for (int ck = 0; ck < NP; ++ck) s += Wm[2][ck][Pk] * bb[cc][ij + NP * NP * ck];
+                        acc[q][c0 + cc] += s;
+                    }
                 }
             }
         }
@@ -224,9 +243,10 @@ __global__ __launch_bounds__(OP_T) void k_l2l(const int32_t *__restrict__ nodes,
                                               const double *__restrict__ ncoord,
                                               double *__restrict__ loc) {
     constexpr int NN = NP * NP * NP;
+    constexpr int CG = TransferGroup<NP, 6>::CG;
     __shared__ double Wm[3][NP][NP];  // [axis][child node][parent node]
     __shared__ double pn[3 * NP], cn[3 * NP];
-    __shared__ double bin[NN], ba[NN], bb[NN];
+    __shared__ double bin[CG][NN], ba[CG][NN], bb[CG][NN];
     const int tid = threadIdx.x;
     const int32_t C = nodes[blockIdx.x];
     const int32_t P = (C - 1) / 2;
@@ -239,29 +259,34 @@ __global__ __launch_bounds__(OP_T) void k_l2l(const int32_t *__restrict__ nodes,
         const int d = tid / NP, ci = tid % NP;
         lgw<NP>(cn[tid], pn + d * NP, &Wm[d][ci][0]);
     }
-    for (int comp = 0; comp < 6; ++comp) {
+    for (int c0 = 0; c0 < 6; c0 += CG) {
         __syncthreads();
-        for (int e = tid; e < NN; e += OP_T) bin[e] = loc[((int64_t)P * NN + e) * 6 + comp];
-        __syncthreads();
-        for (int e = tid; e < NN; e += OP_T) {  // ba[ci, Pj, Pk]
-            const int ci = e % NP, r = e / NP;
-            double s = 0.0;
-            for (int Pi = 0; Pi < NP; ++Pi) s += Wm[0][ci][Pi] * bin[Pi + NP * r];
-            ba[e] = s;
+        for (int e = tid; e < CG * NN; e += OP_T) {  // contiguous when CG = 6
+            const int n = e / CG, cc = e % CG;
+            bin[cc][n] = loc[((int64_t)P * NN + n) * 6 + c0 + cc];
         }
         __syncthreads();
-        for (int e = tid; e < NN; e += OP_T) {  // bb[ci, cj, Pk]
-            const int ci = e % NP, cj = (e / NP) % NP, Pk = e / (NP * NP);
+        for (int e = tid; e < CG * NN; e += OP_T) {  // ba[ci, Pj, Pk]
+            const int cc = e / NN, n = e % NN, ci = n % NP, r = n / NP;
             double s = 0.0;
-            for (int Pj = 0; Pj < NP; ++Pj) s += Wm[1][cj][Pj] * ba[ci + NP * (Pj + NP * Pk)];
-            bb[e] = s;
+            for (int Pi = 0; Pi < NP; ++Pi) s += Wm[0][ci][Pi] * bin[cc][Pi + NP * r];
+            ba[cc][n] = s;
         }
         __syncthreads();
-        for (int e = tid; e < NN; e += OP_T) {
-            const int ij = e % (NP * NP), ck = e / (NP * NP);
+        for (int e = tid; e < CG * NN; e += OP_T) {  // bb[ci, cj, Pk]
+            const int cc = e / NN, n = e % NN;
+            const int ci = n % NP, cj = (n / NP) % NP, Pk = n / (NP * NP);
             double s = 0.0;
-            for (int Pk = 0; Pk < NP; ++Pk) s += Wm[2][ck][Pk] * bb[ij + NP * NP * Pk];
-            loc[((int64_t)C * NN + e) * 6 + comp] += s;
+            for (int Pj = 0; Pj < NP; ++Pj) s += Wm[1][cj][Pj] * ba[cc][ci + NP * (Pj + NP * Pk)];
+            bb[cc][n] = s;
+        }
+        __syncthreads();
+        for (int e = tid; e < CG * NN; e += OP_T) {  // child += contract z (contiguous)
+            const int n = e / CG, cc = e % CG;
+            const int ij = n % (NP * NP), ck = n / (NP * NP);
+            double s = 0.0;
+            for (int Pk = 0; Pk < NP; ++Pk) s += Wm[2][ck][Pk] * bb[cc][ij + NP * NP * Pk];
+            loc[((int64_t)C * NN + n) * 6 + c0 + cc] += s;
         }
     }
 }

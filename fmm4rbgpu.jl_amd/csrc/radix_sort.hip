@@ -2,7 +2,8 @@
 // 32-bit values) for gfx950, 8-bit digits, three kernels per pass:
 //   hist    : per-tile digit histogram (LDS atomics), digit-major table
 //   scan    : per-digit exclusive scan over tiles + digit totals
-//   scatter : stable in-tile ranking with wave64 ballots, then scatter
+//   scatter : stable in-tile ranking with wave64 ballots into LDS (the tile
+//             ordered by digit), then each digit's run written contiguously
 // Several independent arrays of the same length sort in one launch
 // (blockIdx.y selects the array).  Used for the tree's per-axis presort
 // (particle positions) and for grouping interaction pairs by target.
@@ -82,23 +83,33 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
     uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ hist, int nblk,
     const uint32_t *__restrict__ dtot) {
-    __shared__ uint32_t doff[RS_BINS];
+    __shared__ uint32_t doff[RS_BINS];  // global position of this tile's first key of each digit
+    __shared__ uint32_t lst[RS_BINS];   // local position of the same key within the tile
     __shared__ uint32_t run[RS_BINS];
     __shared__ uint32_t wcnt[RS_THREADS / 64][RS_BINS];
     __shared__ uint32_t s[RS_THREADS];
+    __shared__ uint64_t sk[RS_TILE];
+    __shared__ uint32_t sv[HASVAL ? RS_TILE : 1];
     const int seg = blockIdx.y;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t so = (int64_t)seg * n;
 
-    // digit base = exclusive scan of digit totals + this tile's prefix
-    uint32_t tv = dtot[seg * RS_BINS + tid];
-    uint32_t incl = block_incl_scan256(tv, s);
-    doff[tid] = incl - tv + hist[((int64_t)seg * RS_BINS + tid) * nblk + blockIdx.x];
+    // digit base = exclusive scan of digit totals + this tile's prefix; the tile's
+    // own count of each digit = next tile's prefix (or the total) - this prefix
+    const uint32_t tv = dtot[seg * RS_BINS + tid];
+    const uint32_t incl = block_incl_scan256(tv, s);
+    const int64_t hrow = ((int64_t)seg * RS_BINS + tid) * nblk;
+    const uint32_t pre = hist[hrow + blockIdx.x];
+    const uint32_t cnt = (blockIdx.x + 1 < (unsigned)nblk ? hist[hrow + blockIdx.x + 1] : tv) - pre;
+    doff[tid] = incl - tv + pre;
+    const uint32_t lincl = block_incl_scan256(cnt, s);
+    lst[tid] = lincl - cnt;
     run[tid] = 0;
     __syncthreads();
 
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int tn = (int)min<int64_t>(RS_TILE, n - base);
     for (int r = 0; r < RS_ITEMS; ++r) {
         const int64_t i = base + r * RS_THREADS + tid;
         const bool valid = i < n;
@@ -120,9 +131,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
         if (valid) {
             uint32_t off = run[dg] + (uint32_t)__popcll(peers & lt);
             for (int w = 0; w < wave; ++w) off += wcnt[w][dg];
-            const uint32_t pos = doff[dg] + off;
-            kout[so + pos] = key;
-            if (HASVAL) vout[so + pos] = val;
+            const uint32_t lp = lst[dg] + off;  // < tn
+            sk[lp] = key;
+            if (HASVAL) sv[lp] = val;
         }
         __syncthreads();
         uint32_t add = 0;
@@ -130,6 +141,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
         for (int w = 0; w < RS_THREADS / 64; ++w) add += wcnt[w][tid];
         run[tid] += add;
         __syncthreads();
+    }
+    // the tile in digit order: consecutive slots of one digit go to consecutive positions
+    for (int j = tid; j < tn; j += RS_THREADS) {
+        const uint64_t key = sk[j];
+        const uint32_t dg = (uint32_t)(key >> shift) & 0xFFu;
+        const uint32_t pos = doff[dg] + (uint32_t)j - lst[dg];
+        kout[so + pos] = key;
+        if (HASVAL) vout[so + pos] = sv[j];
     }
 }
 
