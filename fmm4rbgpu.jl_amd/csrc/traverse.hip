@@ -10,7 +10,7 @@
 // i, and at most 2*maxlevel+1 iterations run.  Each pair becomes a P2P pair
 // (both leaves), an M2L pair (admissible), or two child pairs (split the leaf
 // -less side, else the side with the larger diameter, ties split the source).
-// Outputs are appended with wave-aggregated atomics as keys (t << sb) | s,
+// Outputs are appended with workgroup-aggregated atomics as keys (t << sb) | s,
 // then radix-sorted, which gives the canonical order (target, then source)
 // and the CSR offsets per target node.
 #include "fmm4rb_internal.hpp"
@@ -49,9 +49,11 @@ __global__ __launch_bounds__(TR_THREADS) void k_traverse(
     uint32_t cap_f, uint64_t *__restrict__ p2p, uint32_t cap_p2p, uint64_t *__restrict__ m2l,
     uint32_t cap_m2l, int sb, const double *__restrict__ geo, const uint8_t *__restrict__ flags,
     const double *__restrict__ pavg4, double eta) {
+    constexpr int NW = TR_THREADS / 64;
+    __shared__ uint32_t wc[3][NW], bb[3];
     const uint32_t n = min(cnt[2 + it], cap_f);
     const double g = pavg4[3];
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (uint32_t base = blockIdx.x * TR_THREADS; base < n; base += gridDim.x * TR_THREADS) {
         const uint32_t i = base + threadIdx.x;
@@ -69,15 +71,28 @@ __global__ __launch_bounds__(TR_THREADS) void k_traverse(
             else cat = gt[4] > gs[4] ? 2 : 3;  // diameter(t) > diameter(s)
         }
         const uint64_t m0 = __ballot(cat == 0), m1 = __ballot(cat == 1), m2 = __ballot(cat >= 2);
-        uint32_t b0 = 0, b1 = 0, b2 = 0;
+        // one atomic per category per workgroup; waves take consecutive ranges
         if (lane == 0) {
-            if (m0) b0 = atomicAdd(&cnt[0], (uint32_t)__popcll(m0));
-            if (m1) b1 = atomicAdd(&cnt[1], (uint32_t)__popcll(m1));
-            if (m2) b2 = atomicAdd(&cnt[2 + it + 1], 2u * (uint32_t)__popcll(m2));
+            wc[0][wave] = (uint32_t)__popcll(m0);
+            wc[1][wave] = (uint32_t)__popcll(m1);
+            wc[2][wave] = 2u * (uint32_t)__popcll(m2);
         }
-        b0 = __shfl(b0, 0);
-        b1 = __shfl(b1, 0);
-        b2 = __shfl(b2, 0);
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            const int k = threadIdx.x;
+            uint32_t tot = 0;
+            for (int w = 0; w < NW; ++w) tot += wc[k][w];
+            uint32_t *ctr = k == 2 ? &cnt[2 + it + 1] : &cnt[k];
+            bb[k] = tot ? atomicAdd(ctr, tot) : 0u;
+        }
+        __syncthreads();
+        uint32_t b0 = bb[0], b1 = bb[1], b2 = bb[2];
+        for (int w = 0; w < wave; ++w) {
+            b0 += wc[0][w];
+            b1 += wc[1][w];
+            b2 += wc[2][w];
+        }
+        __syncthreads();  // wc and bb are rewritten in the next chunk
         if (cat == 0) {
             uint32_t o = b0 + (uint32_t)__popcll(m0 & lt);
             if (o < cap_p2p) p2p[o] = ((uint64_t)t << sb) | (uint64_t)s;
