@@ -273,6 +273,143 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(
     }
 }
 
+// ------------------------------------------------------- deep levels in LDS
+// Once every cluster of a level L0 holds at most TF_CAP particles, one workgroup
+// per cluster finishes its subtree in LDS: for each level l = L0 .. ml the
+// bounding boxes, axes and argmin/argmax particles of the subtree's nodes
+// (as k_level_nodes), and for l < ml the same stable partition of the three rank
+// lists as k_part_scatter; then list x's own ranks (all the leaf sort reads) go
+// back to rk.  At most TF_LEVELS levels below L0 (TF_SUB nodes per level).
+constexpr int TF_CAP = 1024, TF_T = 256, TF_IT = TF_CAP / TF_T;
+constexpr int TF_LEVELS = 6, TF_SUB = 1 << TF_LEVELS;
+
+__global__ __launch_bounds__(TF_T) void k_tree_finish(
+    const int32_t *__restrict__ nodes, int L0, int ml, const int32_t *__restrict__ lo,
+    const int32_t *__restrict__ cnt, const uint8_t *__restrict__ flags, uint32_t *__restrict__ rk,
+    const uint32_t *__restrict__ perm, int64_t N, const double *__restrict__ pos,
+    const double *__restrict__ pavg4, double *__restrict__ bbox, int8_t *__restrict__ axis,
+    int32_t *__restrict__ ext) {
+    __shared__ uint32_t Ls[9][TF_CAP];   // list d, axis e at Ls[3 d + e]
+    __shared__ uint32_t tmp[3][TF_CAP];  // one list's next order
+    __shared__ uint16_t sseg[TF_CAP];    // node (index within the level) of each position
+    __shared__ uint16_t pf[TF_CAP];      // exclusive prefix of the side flags
+    __shared__ uint32_t tsum[TF_T];
+    __shared__ int32_t nlo[TF_SUB], ncnt[TF_SUB];
+    __shared__ uint32_t npiv[TF_SUB];
+    __shared__ int8_t nax[TF_SUB], nspl[TF_SUB];
+    constexpr uint16_t NONE = 0xFFFF;
+    const int tid = threadIdx.x;
+    const int32_t k0 = nodes[blockIdx.x];
+    const int32_t a0 = lo[k0], c0 = cnt[k0];
+    for (int idx = tid; idx < 9 * c0; idx += TF_T) {
+        const int e9 = idx / c0, j = idx - e9 * c0;
+        Ls[e9][j] = rk[(int64_t)e9 * N + a0 + j];
+    }
+    for (int j = tid; j < c0; j += TF_T) sseg[j] = 0;
+    __syncthreads();
+    const double sz = pavg4[3];
+    for (int l = L0; l <= ml; ++l) {
+        const int m = l - L0, nsub = 1 << m;
+        const int64_t first = ((int64_t)k0 + 1) * nsub - 1;
+        for (int u = tid; u < nsub; u += TF_T) {  // clusters.jl:39,51-70 per node
+            const int64_t k = first + u;
+            nspl[u] = 0;
+            ncnt[u] = 0;
+            if (!(flags[k] & 1)) continue;
+            const int32_t b = lo[k] - a0, c = cnt[k];
+            nlo[u] = b;
+            ncnt[u] = c;
+            double mn[3], mx[3];
+            for (int d = 0; d < 3; ++d) {
+                const uint32_t pa = perm[d * N + Ls[4 * d][b]];
+                const uint32_t pb = perm[d * N + Ls[4 * d][b + c - 1]];
+                mn[d] = pos[3 * (int64_t)pa + d] + 0.0;
+                mx[d] = pos[3 * (int64_t)pb + d] + 0.0;
+                bbox[6 * k + d] = mn[d];
+                bbox[6 * k + 3 + d] = mx[d];
+                ext[6 * k + d] = (int32_t)pa;
+                ext[6 * k + 3 + d] = (int32_t)pb;
+            }
+            if ((flags[k] & 3) == 1) {
+                const double sd[3] = {1.0, 1.0, sz};
+                int dir = 0;
+                double best = sd[0] * (mx[0] - mn[0]);
+                for (int d = 1; d < 3; ++d) {
+                    const double v = sd[d] * (mx[d] - mn[d]);
+                    if (v > best) {
+                        best = v;
+                        dir = d;
+                    }
+                }
+                axis[k] = (int8_t)dir;
+                nax[u] = (int8_t)dir;
+                nspl[u] = 1;
+                npiv[u] = Ls[4 * dir][b + ((c + 1) >> 1)];
+            }
+        }
+        __syncthreads();
+        if (l == ml) break;
+        for (int d = 0; d < 3; ++d) {
+            // side flags of this thread's TF_IT consecutive positions, block prefix sum
+            uint32_t f[TF_IT], run = 0;
+#pragma unroll
+            for (int r = 0; r < TF_IT; ++r) {
+                const int j = tid * TF_IT + r;
+                f[r] = 0;
+                if (j < c0) {
+                    const uint16_t u = sseg[j];
+                    if (u != NONE && nspl[u]) f[r] = Ls[3 * d + nax[u]][j] < npiv[u] ? 1u : 0u;
+                }
+                run += f[r];
+            }
+            tsum[tid] = run;
+            __syncthreads();
+            for (int o = 1; o < TF_T; o <<= 1) {
+                const uint32_t v = tid >= o ? tsum[tid - o] : 0u;
+                __syncthreads();
+                tsum[tid] += v;
+                __syncthreads();
+            }
+            uint32_t pre = tsum[tid] - run;
+#pragma unroll
+            for (int r = 0; r < TF_IT; ++r) {
+                const int j = tid * TF_IT + r;
+                if (j < c0) pf[j] = (uint16_t)pre;
+                pre += f[r];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < TF_IT; ++r) {
+                const int j = tid * TF_IT + r;
+                if (j >= c0) continue;
+                const uint16_t u = sseg[j];
+                int np = j;
+                if (u != NONE && nspl[u]) {
+                    const int b = nlo[u], lft = (int)pf[j] - (int)pf[b];
+                    const int nl = (ncnt[u] + 1) >> 1;
+                    np = f[r] ? b + lft : b + nl + (j - b) - lft;
+                }
+#pragma unroll
+                for (int e = 0; e < 3; ++e) tmp[e][np] = Ls[3 * d + e][j];
+            }
+            __syncthreads();
+            for (int j = tid; j < c0; j += TF_T)
+#pragma unroll
+                for (int e = 0; e < 3; ++e) Ls[3 * d + e][j] = tmp[e][j];
+            __syncthreads();
+        }
+        // positions of the next level's nodes (children 2u, 2u + 1); leaves end here
+        for (int j = tid; j < c0; j += TF_T) {
+            const uint16_t u = sseg[j];
+            if (u == NONE) continue;
+            if (nspl[u]) sseg[j] = (uint16_t)(2 * u + ((j - nlo[u]) < ((ncnt[u] + 1) >> 1) ? 0 : 1));
+            else sseg[j] = NONE;
+        }
+        __syncthreads();
+    }
+    for (int j = tid; j < c0; j += TF_T) rk[a0 + j] = Ls[0][j];
+}
+
 // Canonical within-leaf order: ascending particle id (rank sort in LDS).
 __global__ __launch_bounds__(256) void k_leaf_sort(const int32_t *__restrict__ leaves,
                                                    const int32_t *__restrict__ lo,
@@ -342,7 +479,18 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
     const uint8_t *flags = c.d_flags.as<uint8_t>();
     int8_t *axis = c.d_axis.as<int8_t>();
     uint32_t *pivot = c.d_pivot.as<uint32_t>();
-    for (int l = 0; l <= S.ml; ++l) {
+    // first level whose clusters all fit the LDS finish (at most TF_LEVELS above ml)
+    int L0 = S.ml;
+    for (int l = std::max(0, S.ml - TF_LEVELS); l <= S.ml; ++l) {
+        int mx = 0;
+        for (int64_t q = S.lvl_off[l]; q < S.lvl_off[l + 1]; ++q)
+            mx = std::max(mx, S.cnt[S.lvl_nodes[q]]);
+        if (mx <= TF_CAP) {
+            L0 = l;
+            break;
+        }
+    }
+    for (int l = 0; l < L0; ++l) {
         int nnodes = (int)(S.lvl_off[l + 1] - S.lvl_off[l]);
         if (nnodes > 0) {
             hipLaunchKernelGGL(k_level_nodes, dim3(nb(nnodes, 128)), dim3(128), 0, st, nnodes,
@@ -352,7 +500,7 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
             FMM_LAUNCH_CHECK();
         }
         int nsplit = (int)(S.split_off[l + 1] - S.split_off[l]);
-        if (l == S.ml || nsplit == 0) continue;
+        if (nsplit == 0) continue;
         hipLaunchKernelGGL(k_part_count, dim3(nblk, 3), dim3(PT_THREADS), 0, st, N, l, seg, flags,
                            axis, pivot, rk, c.d_bsum.as<uint32_t>(), nblk);
         FMM_LAUNCH_CHECK();
@@ -363,6 +511,14 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
         FMM_LAUNCH_CHECK();
         std::swap(rk, rk_alt);
         std::swap(seg, seg_alt);
+    }
+    {
+        const int nnodes = (int)(S.lvl_off[L0 + 1] - S.lvl_off[L0]);
+        hipLaunchKernelGGL(k_tree_finish, dim3(nnodes), dim3(TF_T), 0, st,
+                           c.d_lvl_nodes.as<int32_t>() + S.lvl_off[L0], L0, S.ml, lo, cnt, flags,
+                           rk, perm, N, pos, c.d_pavg.as<double>(), c.d_bbox.as<double>(), axis,
+                           c.d_ext.as<int32_t>());
+        FMM_LAUNCH_CHECK();
     }
     const int nleaf = (int)S.leaves.size();
     hipLaunchKernelGGL(k_leaf_sort, dim3(nleaf), dim3(256), 0, st, c.d_leaves.as<int32_t>(), lo,
