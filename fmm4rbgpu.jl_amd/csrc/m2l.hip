@@ -46,30 +46,41 @@ __device__ inline float rsq<float>(float d) { return __builtin_amdgcn_rsqf(d); }
 template <int NP, typename T, typename V4>
 __device__ inline void m2l_pair(double tx, double ty, double tz, const double *sn, const V4 *smp,
                                 T px, T py, T pz, double *acc) {
-    T dxr[NP], dx2[NP], dyr[NP], dy2[NP];
+    // FP64 with n > 5: squares formed in the multiply-adds and rows not unrolled
+    // (122 instead of 164 VGPRs: FP64 M2L at n = 6 took 307 ms instead of 421 ms at
+    // N = 2^22); otherwise precomputed squares and unrolled rows (faster there).
+    constexpr bool LEAN = sizeof(T) == 8 && NP > 6;
+    constexpr int ROWU = LEAN ? 1 : NP;
+    T dxr[NP], dyr[NP], dx2[LEAN ? 1 : NP], dy2[LEAN ? 1 : NP];
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
         dxr[i] = (T)(tx - sn[i]);
-        dx2[i] = dxr[i] * dxr[i];
         dyr[i] = (T)(ty - sn[NP + i]);
-        dy2[i] = fma(dyr[i], dyr[i], (T)EPS64);
+        if constexpr (!LEAN) {
+            dx2[i] = dxr[i] * dxr[i];
+            dy2[i] = fma(dyr[i], dyr[i], (T)EPS64);
+        }
     }
     T H = 0, Uy = 0, Uz = 0, e1 = 0, e2 = 0, b0 = 0, b1 = 0, b2 = 0;
     for (int sk = 0; sk < NP; ++sk) {
         const T dz = (T)(tz - sn[2 * NP + sk]);
-        const T dz2 = dz * dz, pzdz = pz * dz;
+        const T dz2 = LEAN ? fma(dz, dz, (T)EPS64) : dz * dz, pzdz = pz * dz;
         T Gp = 0, Axp = 0, Ayp = 0;
-#pragma unroll
+#pragma unroll ROWU
         for (int sj = 0; sj < NP; ++sj) {
             const T dy = dyr[sj];
-            const T rrjk = dy2[sj] + dz2;
+            T rrjk;
+            if constexpr (LEAN) rrjk = fma(dy, dy, dz2);
+            else rrjk = dy2[sj] + dz2;
             const T prjk = fma(py, dy, pzdz);
             const V4 *row = smp + NP * (sj + NP * sk);
             T G, Ax, Az;
 #pragma unroll
             for (int si = 0; si < NP; ++si) {
                 const T dx = dxr[si];
-                const T rr = dx2[si] + rrjk;
+                T rr;
+                if constexpr (LEAN) rr = fma(dx, dx, rrjk);
+                else rr = dx2[si] + rrjk;
                 const T pr = fma(px, dx, prjk);
                 const T y = rsq<T>(fma(pr, pr, rr));
                 const T y3 = (y * y) * y;
