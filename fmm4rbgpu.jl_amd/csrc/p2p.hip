@@ -38,16 +38,13 @@ __device__ inline void p2p_pair(const double *xi, double sx, double sy, double s
     if constexpr (sizeof(T) == 4) y = __builtin_amdgcn_rsqf(d);
     else y = rsqrt_nr(d);
     const T y3 = (y * y) * y;
-    const T gw = g * y3;
-    a[0] = fma(gw, R0, a[0]);
-    a[1] = fma(gw, R1, a[1]);
-    a[2] = fma(gw, R2, a[2]);
-    const T c0 = fma(py, R2, -(pz * R1));
-    const T c1 = fma(pz, R0, -(px * R2));
-    const T c2 = fma(px, R1, -(py * R0));
-    a[3] = fma(y3, c0, a[3]);
-    a[4] = fma(y3, c1, a[4]);
-    a[5] = fma(y3, c2, a[5]);
+    const T u0 = y3 * R0, u1 = y3 * R1, u2 = y3 * R2;  // K = d^-3/2 R
+    a[0] = fma(g, u0, a[0]);                            // E += g K
+    a[1] = fma(g, u1, a[1]);
+    a[2] = fma(g, u2, a[2]);
+    a[3] = fma(-pz, u1, fma(py, u2, a[3]));             // B += p x K
+    a[4] = fma(-px, u2, fma(pz, u0, a[4]));
+    a[5] = fma(-py, u0, fma(px, u1, a[5]));
 }
 
 template <bool MIXED, int TPL>

@@ -293,7 +293,7 @@ __global__ __launch_bounds__(TF_T) void k_tree_finish(
     __shared__ uint32_t tmp[3][TF_CAP];  // one list's next order
     __shared__ uint16_t sseg[TF_CAP];    // node (index within the level) of each position
     __shared__ uint16_t pf[TF_CAP];      // exclusive prefix of the side flags
-    __shared__ uint32_t tsum[TF_T];
+    __shared__ uint32_t tsum[TF_T / 64];
     __shared__ int32_t nlo[TF_SUB], ncnt[TF_SUB];
     __shared__ uint32_t npiv[TF_SUB];
     __shared__ int8_t nax[TF_SUB], nspl[TF_SUB];
@@ -362,15 +362,17 @@ __global__ __launch_bounds__(TF_T) void k_tree_finish(
                 }
                 run += f[r];
             }
-            tsum[tid] = run;
-            __syncthreads();
-            for (int o = 1; o < TF_T; o <<= 1) {
-                const uint32_t v = tid >= o ? tsum[tid - o] : 0u;
-                __syncthreads();
-                tsum[tid] += v;
-                __syncthreads();
+            // block prefix sum: wave scans by shuffles, then the waves' totals
+            uint32_t x = run;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o);
+                if ((tid & 63) >= o) x += y;
             }
-            uint32_t pre = tsum[tid] - run;
+            if ((tid & 63) == 63) tsum[tid >> 6] = x;
+            __syncthreads();
+            uint32_t pre = x - run;
+            for (int w = 0; w < (tid >> 6); ++w) pre += tsum[w];
 #pragma unroll
             for (int r = 0; r < TF_IT; ++r) {
                 const int j = tid * TF_IT + r;
