@@ -493,19 +493,38 @@ struct AShape {
 // k_m2l) and stores them in loc.  With MIXED the node differences are formed in
 // FP64 and rounded once, the terms run in FP32 (v_rsq_f32) and every source
 // cluster's FP32 sums are added into FP64 (mp holds float4 multipoles then).
+//
+// Source-node coordinates (NP <= 6): the source cluster's dl/dY_s[b] is minus the
+// same per-pair term J^T u summed over the target nodes, and only its sums over
+// the nodes with index m on axis d are needed.  Per source cluster s, each lane
+// groups its terms by the source node's x index (column sums), y index (row
+// sums) and z index (plane sums); the block adds the lanes' 3 NP values through
+// LDS and writes them to ysp at the pair's position in the transposed list
+// (posT), so that the source side only reads them.
+template <int NP>
+struct YSplit {
+    static constexpr bool on = NP <= 6;  // 3 NP x BLK doubles of LDS <= 36 KB
+};
+
 template <int NP, bool FWD, bool MIXED>
 __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
     const int32_t *__restrict__ off, const uint64_t *__restrict__ keys, int sb,
     const double *__restrict__ ncoord, const void *__restrict__ mpv,
     const double *__restrict__ lbar, const double *__restrict__ pavg4, double *__restrict__ Xbar,
-    double *__restrict__ pbar_part, double *__restrict__ loc) {
+    double *__restrict__ pbar_part, double *__restrict__ loc, const uint32_t *__restrict__ posT,
+    double *__restrict__ ysp) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using V4 = typename std::conditional<MIXED, float4, double4>::type;
     constexpr int NN = AShape<NP>::NN, BLK = AShape<NP>::BLK, TPT = AShape<NP>::TPT;
     constexpr int SH = (4 * NN > 3 * BLK) ? 4 * NN : 3 * BLK;
     constexpr int NA = 17;  // accumulators: F[6], G, A[3], sum c R [3], sum c pr, sum c pr R [3]
+    constexpr bool YS = YSplit<NP>::on;
+    static_assert(!YS || TPT == 1, "source-node sums assume one target node per thread");
+    constexpr int NY = YS ? 3 * NP : 1, KS = YS ? BLK / 16 : 1;
     __shared__ double shb[SH];
     __shared__ double sn[3 * NP];
+    __shared__ double ybuf[NY][YS ? BLK : 1];  // per lane: column, row, plane sums
+    __shared__ double ybr[NY][KS];             // sums over 16 lanes
     V4 *smp = reinterpret_cast<V4 *>(shb);
     double(*yb)[3] = reinterpret_cast<double(*)[3]>(shb);
     double(*pr3)[3] = reinterpret_cast<double(*)[3]>(shb);
@@ -517,6 +536,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
     const T px = (T)pavg4[0], py = (T)pavg4[1], pz = (T)pavg4[2];
     double tx[TPT], ty[TPT], tz[TPT];
     T e[TPT][3], b[TPT][3], ac[TPT][NA];
+    T u3c[TPT][3];       // e0/3, b1/3, b2/3: (J^T u)_x / 3 per pair for the column sums
     double ad[TPT][NA];  // FP64 sums (MIXED)
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
@@ -529,8 +549,13 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
             e[q][d] = (T)lb[d];
             b[q][d] = (T)lb[3 + d];
         }
+        u3c[q][0] = (T)(lb[0] / 3.0);
+        u3c[q][1] = (T)(lb[4] / 3.0);
+        u3c[q][2] = (T)(lb[5] / 3.0);
         for (int k = 0; k < NA; ++k) ac[q][k] = 0, ad[q][k] = 0.0;
     }
+    if (YS && tid >= NN)
+        for (int j = 0; j < NY; ++j) ybuf[j][tid] = 0.0;
     for (int32_t ei = beg; ei < end; ++ei) {
         const int64_t s = (int64_t)(keys[ei] & mask);
         __syncthreads();
@@ -544,10 +569,13 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
             const T e0 = e[q][0], e1 = e[q][1], e2 = e[q][2];
             const T b0 = b[q][0], b1 = b[q][1], b2 = b[q][2];
             T dxv[NP], dx2v[NP];
+            T Xs[NP];  // column sums / 3 (source x index); row sums in ybuf[NP + y index]
 #pragma unroll
             for (int si = 0; si < NP; ++si) {
                 dxv[si] = (T)(tx[q] - sn[si]);
                 dx2v[si] = dxv[si] * dxv[si];
+                Xs[si] = 0;
+                if (YS) ybuf[NP + si][tid] = 0.0;
             }
             // forward sums as in m2l_pair: H, U_y, U_z run over the source cluster;
             // per plane G, A_x, A_y, A_z (also the totals of the adjoint's e G + b x A);
@@ -556,7 +584,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
 #pragma unroll 1
             for (int sk = 0; sk < NP; ++sk) {
                 const T dz = (T)(tz[q] - sn[2 * NP + sk]);
-                T Gp = 0, Axp = 0, Ayp = 0, Azp = 0;
+                T Gp = 0, Axp = 0, Ayp = 0, Azp = 0, C0p = 0, Cpp = 0;
                 for (int sj = 0; sj < NP; ++sj) {
                     const T dy = (T)(ty[q] - sn[NP + sj]);
                     const T rrjk = fma(dy, dy, dz * dz) + (T)EPS64;
@@ -600,6 +628,10 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                             Uy = fma(uu, m.z, Uy);
                             Uz = fma(uu, m.w, Uz);
                         }
+                        if constexpr (YS) {  // (J^T u)_x / 3 = y3 u_x / 3 - c (dx + pr px)
+                            const T u3 = fma(m.x, u3c[q][0], fma(u3c[q][1], m.w, -(u3c[q][2] * m.z)));
+                            Xs[si] = fma(y3, u3, fma(-c, fma(pr, px, dx), Xs[si]));
+                        }
                     }
                     // pr = px dx + prjk along the row: sum c pr and sum c pr dx from C0..C2
                     const T Cp = fma(px, C1, prjk * C0), Cpx = fma(px, C2, prjk * C1);
@@ -607,6 +639,12 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                         A[1] = fma(dy, G, A[1]);   // E_y
                         A[3] = fma(-dy, Az, A[3]);  // B_x (dy part)
                         A[5] = fma(dy, Ax, A[5]);   // B_z (dy part)
+                    }
+                    if constexpr (YS) {  // row: (J^T u)_y = e1 G + (b x A)_y - 3 (dy C0 + py Cp)
+                        ybuf[NP + sj][tid] += (double)(fma(e1, G, fma(b2, Ax, -(b0 * Az))) -
+                                                       (T)3 * fma(dy, C0, py * Cp));
+                        C0p += C0;
+                        Cpp += Cp;
                     }
                     Gp += G;
                     Axp += Ax;
@@ -624,6 +662,9 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                     A[3] = fma(dz, Ayp, A[3]);  // B_x (dz part)
                     A[4] = fma(-dz, Axp, A[4]);  // B_y (dz part)
                 }
+                if constexpr (YS)  // plane: (J^T u)_z = e2 G + (b x A)_z - 3 (dz C0 + pz Cp)
+                    ybuf[2 * NP + sk][tid] = (double)(fma(e2, Gp, fma(b0, Ayp, -(b1 * Axp))) -
+                                                      (T)3 * fma(dz, C0p, pz * Cpp));
                 A[6] += Gp;
                 A[7] += Axp;
                 A[8] += Ayp;
@@ -636,6 +677,26 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
             }
             if constexpr (MIXED)
                 for (int k = 0; k < NA; ++k) ad[q][k] += (double)A[k], A[k] = 0;
+            if constexpr (YS) {
+#pragma unroll
+                for (int si = 0; si < NP; ++si) ybuf[si][tid] = 3.0 * (double)Xs[si];
+            }
+        }
+        if constexpr (YS) {  // lanes -> sums over 16 lanes; finished at the next pair
+            __syncthreads();
+            for (int w = tid; w < NY * KS; w += BLK) {
+                const int j = w / KS, k = w % KS;
+                double v = 0.0;
+#pragma unroll
+                for (int l = 0; l < 16; ++l) v += ybuf[j][16 * k + l];
+                ybr[j][k] = v;
+            }
+            __syncthreads();
+            if (tid < NY) {
+                double v = 0.0;
+                for (int k = 0; k < KS; ++k) v += ybr[tid][k];
+                ysp[(int64_t)posT[ei] * NY + tid] = v;
+            }
         }
     }
     double pd[3] = {0.0, 0.0, 0.0};
@@ -701,17 +762,21 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
 
 // Source side: per source node b, over the targets that list it: Mbar_s[b]
 // (= dl/dgamma, dl/dm) and dl/dY_s[b].  MIXED as in k_adj_m2l_t (the target
-// clusters' local cotangents are staged as FP32).
+// clusters' local cotangents are staged as FP32).  For NP <= 6 the target side
+// has already summed dl/dY_s per pair (ysp, in transposed-list order): only
+// the multipole cotangents are evaluated here, with 8 sums per node pair.
 template <int NP, bool MIXED>
 __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
     const int32_t *__restrict__ offT, const uint64_t *__restrict__ keysT, int sb,
     const double *__restrict__ ncoord, const void *__restrict__ mpv,
     const double *__restrict__ lbar, const double *__restrict__ pavg4, double *__restrict__ mbar,
-    double *__restrict__ Xbar) {
+    double *__restrict__ Xbar, const double *__restrict__ ysp) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using V4 = typename std::conditional<MIXED, float4, double4>::type;
     constexpr int NN = AShape<NP>::NN, BLK = AShape<NP>::BLK, TPT = AShape<NP>::TPT;
-    constexpr int NA = 14;  // gb[4], sum y3 e [3], sum y3 b [3], sum c R [3], sum c pr
+    constexpr bool YS = YSplit<NP>::on;
+    // gb[4]; without YS also sum y3 e [3], sum y3 b [3], sum c R [3], sum c pr
+    constexpr int NA = YS ? 4 : 14;
     __shared__ double shl[NN * 6];
     __shared__ double tn[3 * NP];
     T *slb = reinterpret_cast<T *>(shl);
@@ -753,7 +818,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
                 const T dz = (T)(tn[2 * NP + tk] - sz[q]);
                 for (int tj = 0; tj < NP; ++tj) {
                     const T dy = (T)(tn[NP + tj] - sy[q]);
-                    const T rrjk = fma(dy, dy, dz * dz) + (T)EPS64;
+                    const T rrjk = fma(dy, dy, fma(dz, dz, (T)EPS64));
                     const T prjk = fma(py, dy, pz * dz);
                     const T w0 = fma(m.z, dz, -(m.w * dy));  // (m x R)_0
                     const T w1r = -(m.y * dz), w2r = m.y * dy;
@@ -771,7 +836,6 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
                         const T *o = slb + 6 * (ti + NP * (tj + NP * tk));
                         const T e0 = o[0], e1 = o[1], e2 = o[2], b0 = o[3], b1 = o[4], b2 = o[5];
                         Sxe = fma(uu, e0, Sxe);
-                        Ex = fma(y3, e0, Ex);
                         Ey = fma(y3, e1, Ey);
                         Ez = fma(y3, e2, Ez);
                         Bx = fma(y3, b0, Bx);
@@ -779,33 +843,52 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
                         Bz = fma(y3, b2, Bz);
                         Sxby = fma(uu, b1, Sxby);
                         Sxbz = fma(uu, b2, Sxbz);
-                        const T w1 = fma(m.w, dx, w1r), w2 = fma(-m.z, dx, w2r);
-                        const T Ru = fma(m.x, fma(dx, e0, fma(dy, e1, dz * e2)),
-                                         fma(b0, w0, fma(b1, w1, b2 * w2)));
-                        const T c = y3 * (y2 * Ru);
-                        C0 += c;
-                        C1 = fma(c, dx, C1);
+                        if constexpr (!YS) {
+                            Ex = fma(y3, e0, Ex);
+                            const T w1 = fma(m.w, dx, w1r), w2 = fma(-m.z, dx, w2r);
+                            const T Ru = fma(m.x, fma(dx, e0, fma(dy, e1, dz * e2)),
+                                             fma(b0, w0, fma(b1, w1, b2 * w2)));
+                            const T c = y3 * (y2 * Ru);
+                            C0 += c;
+                            C1 = fma(c, dx, C1);
+                        }
                     }
-                    const T Cp = fma(px, C1, prjk * C0);  // sum c pr (pr = px dx + prjk)
                     A[0] += fma(dy, Ey, fma(dz, Ez, Sxe));  // dl/dgamma = sum d^-3/2 e.R
-                    A[1] += dy * Bz - dz * By;              // dl/dm = sum d^-3/2 R x b
-                    A[2] += dz * Bx - Sxbz;
-                    A[3] += Sxby - dy * Bx;
-                    A[4] += Ex;
-                    A[5] += Ey;
-                    A[6] += Ez;
-                    A[7] += Bx;
-                    A[8] += By;
-                    A[9] += Bz;
-                    A[10] += C1;
-                    A[11] = fma(dy, C0, A[11]);
-                    A[12] = fma(dz, C0, A[12]);
-                    A[13] += Cp;
+                    A[1] += fma(dy, Bz, -(dz * By));        // dl/dm = sum d^-3/2 R x b
+                    A[2] += fma(dz, Bx, -Sxbz);
+                    A[3] += fma(-dy, Bx, Sxby);
+                    if constexpr (!YS) {
+                        const T Cp = fma(px, C1, prjk * C0);  // sum c pr (pr = px dx + prjk)
+                        A[4] += Ex;
+                        A[5] += Ey;
+                        A[6] += Ez;
+                        A[7] += Bx;
+                        A[8] += By;
+                        A[9] += Bz;
+                        A[10] += C1;
+                        A[11] = fma(dy, C0, A[11]);
+                        A[12] = fma(dz, C0, A[12]);
+                        A[13] += Cp;
+                    }
                 }
             }
             if constexpr (MIXED)
                 for (int k = 0; k < NA; ++k) ad[q][k] += (double)A[k], A[k] = 0;
         }
+    }
+    if constexpr (YS) {
+        const int node = tid;
+        if (node < NN) {
+            const double *S = MIXED ? ad[0] : nullptr;
+            double *o = mbar + ((int64_t)s * NN + node) * 4;
+            for (int d = 0; d < 4; ++d) o[d] = MIXED ? S[d] : (double)ac[0][d];
+        }
+        if (tid < 3 * NP) {  // dl/dY_s = - sum over the pairs of the target side's sums
+            double sum = 0.0;
+            for (int32_t ei = beg; ei < end; ++ei) sum += ysp[(int64_t)ei * 3 * NP + tid];
+            Xbar[(int64_t)s * 3 * NP + tid] -= sum;
+        }
+        return;
     }
     __syncthreads();
 #pragma unroll
@@ -860,7 +943,7 @@ __device__ inline T pair_geo(double R0d, double R1d, double R2d, T px, T py, T p
     R[1] = (T)R1d;
     R[2] = (T)R2d;
     pr = fma(px, R[0], fma(py, R[1], pz * R[2]));
-    const T d = fma(pr, pr, fma(R[0], R[0], fma(R[1], R[1], R[2] * R[2]))) + (T)EPS64;
+    const T d = fma(pr, pr, fma(R[0], R[0], fma(R[1], R[1], fma(R[2], R[2], (T)EPS64))));
     T y;
     if constexpr (sizeof(T) == 4) y = __builtin_amdgcn_rsqf(d);
     else y = rsqrt_nr(d);
@@ -868,6 +951,35 @@ __device__ inline T pair_geo(double R0d, double R1d, double R2d, T px, T py, T p
 }
 
 // Target side: gx_i += sum_j J^T u_ij,  u_ij = g_j e_i + b_i x p_j.
+// One staged tile for the lane's PNG targets; DIAG: the tile holds the targets'
+// own leaf, and the self pair (source k == selfk + 16 q) is skipped.
+template <bool DIAG, typename T, typename V4>
+__device__ __forceinline__ void adj_t_tile(const double4 *sx, const V4 *sq, int g, int m,
+                                           int selfk, const double (&xi)[PNG][3],
+                                           const T (&e)[PNG][3], const T (&b)[PNG][3],
+                                           T (&part)[PNG][3]) {
+    for (int k = g; k < m; k += PNG) {
+        const double4 xs = sx[k];
+        const V4 sqk = sq[k];  // gamma, px, py, pz
+#pragma unroll
+        for (int q = 0; q < PNG; ++q) {
+            T R[3], pr;
+            const bool self = DIAG && k == selfk + 16 * q;
+            const T y = pair_geo<T>(xi[q][0] - xs.x, xi[q][1] - xs.y, xi[q][2] - xs.z, sqk.y,
+                                    sqk.z, sqk.w, self, R, pr);
+            const T y2 = y * y, y3 = y2 * y;
+            const T u0 = fma(sqk.x, e[q][0], fma(b[q][1], sqk.w, -(b[q][2] * sqk.z)));
+            const T u1 = fma(sqk.x, e[q][1], fma(b[q][2], sqk.y, -(b[q][0] * sqk.w)));
+            const T u2 = fma(sqk.x, e[q][2], fma(b[q][0], sqk.z, -(b[q][1] * sqk.y)));
+            const T Ru = fma(R[0], u0, fma(R[1], u1, R[2] * u2));
+            const T c3 = (T)3 * (y3 * (y2 * Ru));
+            part[q][0] = fma(y3, u0, fma(-c3, fma(pr, sqk.y, R[0]), part[q][0]));
+            part[q][1] = fma(y3, u1, fma(-c3, fma(pr, sqk.z, R[1]), part[q][1]));
+            part[q][2] = fma(y3, u2, fma(-c3, fma(pr, sqk.w, R[2]), part[q][2]));
+        }
+    }
+}
+
 template <bool MIXED>
 __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ leaves,
                                                   const int32_t *__restrict__ off,
@@ -923,26 +1035,10 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ le
                 T part[PNG][3];
 #pragma unroll
                 for (int q = 0; q < PNG; ++q) part[q][0] = part[q][1] = part[q][2] = 0;
-                for (int k = g; k < m; k += PNG) {
-                    const double4 xs = sx[k];
-                    const V4 sqk = sq[k];  // gamma, px, py, pz
-#pragma unroll
-                    for (int q = 0; q < PNG; ++q) {
-                        T R[3], pr;
-                        const bool self = t == s && tb + wbase + li + 16 * q == s0 + k;
-                        const T y = pair_geo<T>(xi[q][0] - xs.x, xi[q][1] - xs.y, xi[q][2] - xs.z,
-                                                sqk.y, sqk.z, sqk.w, self, R, pr);
-                        const T y2 = y * y, y3 = y2 * y;
-                        const T u0 = fma(sqk.x, e[q][0], b[q][1] * sqk.w - b[q][2] * sqk.z);
-                        const T u1 = fma(sqk.x, e[q][1], b[q][2] * sqk.y - b[q][0] * sqk.w);
-                        const T u2 = fma(sqk.x, e[q][2], b[q][0] * sqk.z - b[q][1] * sqk.y);
-                        const T Ru = fma(R[0], u0, fma(R[1], u1, R[2] * u2));
-                        const T c3 = (T)3 * (y3 * (y2 * Ru));
-                        part[q][0] += y3 * u0 - c3 * fma(pr, sqk.y, R[0]);
-                        part[q][1] += y3 * u1 - c3 * fma(pr, sqk.z, R[1]);
-                        part[q][2] += y3 * u2 - c3 * fma(pr, sqk.w, R[2]);
-                    }
-                }
+                if (t == s)
+                    adj_t_tile<true>(sx, sq, g, m, tb + wbase + li - s0, xi, e, b, part);
+                else
+                    adj_t_tile<false>(sx, sq, g, m, 0, xi, e, b, part);
 #pragma unroll
                 for (int q = 0; q < PNG; ++q)
 #pragma unroll
@@ -1043,6 +1139,11 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
                     for (int q = 0; q < NG; ++q)
 #pragma unroll
                         for (int d = 0; d < NA; ++d) part[q][d] = 0;
+                // the self pair can only occur when the staged leaf is the source leaf:
+                // MIXED runs a check-free copy of the loop on the other leaves (FP64
+                // keeps one copy, which needs fewer registers)
+                const int selfk = s == t ? sb0 + wbase + li - t0 : -(1 << 30);
+                auto tile = [&](auto diag) {
                 for (int k = g; k < m; k += NG) {
                     const double4 xi = tx[k];
                     const V4 e = te[k], bb = tb4[k];
@@ -1050,7 +1151,7 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
                     for (int q = 0; q < NG; ++q) {
                         const T px = ps[q][0], py = ps[q][1], pz = ps[q][2];
                         T R[3], pr;
-                        const bool self = s == t && sb0 + wbase + li + TL * q == t0 + k;
+                        const bool self = decltype(diag)::value && k == selfk + TL * q;
                         const T y = pair_geo<T>(xi.x - xs[q][0], xi.y - xs[q][1], xi.z - xs[q][2],
                                                 px, py, pz, self, R, pr);
                         const T y2 = y * y, y3 = y2 * y;
@@ -1083,6 +1184,9 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
                         P[16] = fma(cp, R[2], P[16]);
                     }
                 }
+                };
+                if (!MIXED || s == t) tile(std::true_type{});
+                else tile(std::false_type{});
                 if constexpr (MIXED) {
 #pragma unroll
                     for (int q = 0; q < NG; ++q)
@@ -1232,8 +1336,6 @@ __global__ __launch_bounds__(256) void k_adj_pbar(const double *__restrict__ par
 
 static inline unsigned nbk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
-void transpose_lists(Ctx &c, const uint64_t *sorted, int64_t n, DevBuf &kT, DevBuf &kT_alt,
-                     DevBuf &offT, uint64_t **sortedT);
 
 void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, double *gpos,
                  double *gmom) {
@@ -1255,7 +1357,13 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     FMM_HIP(hipMemcpyAsync(c.d_costab.p, c.costab_host, sizeof c.costab_host, hipMemcpyHostToDevice,
                            st));
     transpose_lists(c, c.p2p_sorted, c.np2p, c.d_p2pT, c.d_p2pT_alt, c.d_p2pT_off, &c.p2pT_sorted);
-    transpose_lists(c, c.m2l_sorted, c.nm2l, c.d_m2lT, c.d_m2lT_alt, c.d_m2lT_off, &c.m2lT_sorted);
+    // the target side's per-pair source-node sums (n <= 5) go to the transposed position
+    const bool ysplit = c.np <= 6;
+    transpose_lists(c, c.m2l_sorted, c.nm2l, c.d_m2lT, c.d_m2lT_alt, c.d_m2lT_off, &c.m2lT_sorted,
+                    ysplit ? &c.d_m2lT_pos : nullptr);
+    if (ysplit) c.d_ysp.ensure(3 * (size_t)c.np * (size_t)std::max<int64_t>(c.nm2l, 1) * 8);
+    const uint32_t *posT = ysplit ? c.d_m2lT_pos.as<uint32_t>() : nullptr;
+    double *ysp = ysplit ? c.d_ysp.as<double>() : nullptr;
 
     double *ob = c.d_ob.as<double>(), *gx = c.d_gx.as<double>(), *gp = c.d_gp.as<double>();
     double *lbar = c.d_lbar.as<double>(), *mbar = c.d_mbar.as<double>();
@@ -1297,11 +1405,13 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     if (mixed)                                                                                     \
         hipLaunchKernelGGL((k_adj_m2l_t<NP, true, true>), dim3((unsigned)nc),                      \
                            dim3(AShape<NP>::BLK), 0, st, c.d_m2l_off.as<int32_t>(), c.m2l_sorted,  \
-                           c.sb, ncoord, mpv, lbar, c.d_pavg.as<double>(), xbar, pbar, loc);       \
+                           c.sb, ncoord, mpv, lbar, c.d_pavg.as<double>(), xbar, pbar, loc, posT, \
+                           ysp);                                                                   \
     else                                                                                           \
         hipLaunchKernelGGL((k_adj_m2l_t<NP, true, false>), dim3((unsigned)nc),                     \
                            dim3(AShape<NP>::BLK), 0, st, c.d_m2l_off.as<int32_t>(), c.m2l_sorted,  \
-                           c.sb, ncoord, mpv, lbar, c.d_pavg.as<double>(), xbar, pbar, loc)
+                           c.sb, ncoord, mpv, lbar, c.d_pavg.as<double>(), xbar, pbar, loc, posT, \
+                           ysp)
     ADJ_NP_SWITCH(c.np, ADJ_M2LT);
     FMM_LAUNCH_CHECK();
     launch_l2l(c);
@@ -1327,11 +1437,11 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     if (mixed)                                                                                    \
         hipLaunchKernelGGL((k_adj_m2l_s<NP, true>), dim3((unsigned)nc), dim3(AShape<NP>::BLK), 0, \
                            st, c.d_m2lT_off.as<int32_t>(), c.m2lT_sorted, c.sb, ncoord, mpv, lbar, \
-                           c.d_pavg.as<double>(), mbar, xbar);                                    \
+                           c.d_pavg.as<double>(), mbar, xbar, ysp);                               \
     else                                                                                          \
         hipLaunchKernelGGL((k_adj_m2l_s<NP, false>), dim3((unsigned)nc), dim3(AShape<NP>::BLK),   \
                            0, st, c.d_m2lT_off.as<int32_t>(), c.m2lT_sorted, c.sb, ncoord, mpv,    \
-                           lbar, c.d_pavg.as<double>(), mbar, xbar)
+                           lbar, c.d_pavg.as<double>(), mbar, xbar, ysp)
     ADJ_NP_SWITCH(c.np, ADJ_M2LS);
     FMM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_adj_pbar, dim3(1), dim3(256), 0, st, pbar, nc, pbar + 3 * nc);

@@ -113,8 +113,11 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
 
 // ---- traversal (traverse.hip)
 void build_lists_device(Ctx &c, double eta);
+// Sort (target << sb | source) keys, CSR offsets per target; vals (may be null)
+// are permuted alongside, *vsorted = the buffer holding them sorted.
 void sort_pairs_csr(Ctx &c, uint64_t *keys, uint64_t *keys_alt, int64_t n, int32_t *off,
-                    uint64_t **sorted);
+                    uint64_t **sorted, uint32_t *vals = nullptr, uint32_t *vals_alt = nullptr,
+                    uint32_t **vsorted = nullptr);
 
 // ---- interpolation operators (interp.hip)
 void launch_node_geo(Ctx &c);
@@ -132,7 +135,7 @@ void launch_p2p(Ctx &c, const int32_t *off, const uint64_t *keys, int sb);
 void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, double *gpos,
                  double *gmom);
 void transpose_lists(Ctx &c, const uint64_t *sorted, int64_t n, DevBuf &kT, DevBuf &kT_alt,
-                     DevBuf &offT, uint64_t **sortedT);
+                     DevBuf &offT, uint64_t **sortedT, DevBuf *pos = nullptr);
 void launch_bruteforce_vjp(const double *pos, const double *mom, int64_t N, const double *ob,
                            double *gpos, double *gmom, hipStream_t st);
 
@@ -173,6 +176,7 @@ struct Ctx {
     // adjoint (field_vjp)
     DevBuf d_ob, d_gx, d_gp, d_lbar, d_mbar, d_xbar, d_pbar, d_abar;
     DevBuf d_p2pT, d_p2pT_alt, d_m2lT, d_m2lT_alt, d_p2pT_off, d_m2lT_off;
+    DevBuf d_m2lT_pos, d_ysp;  // transposed position of each M2L pair; per-pair source sums
     uint64_t *p2pT_sorted = nullptr, *m2lT_sorted = nullptr;
     // debug snapshots
     std::vector<double> snap_loc_m2l;

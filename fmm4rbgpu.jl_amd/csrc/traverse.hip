@@ -130,40 +130,65 @@ __global__ void k_csr(const uint64_t *__restrict__ keys, int64_t n, int sb, int6
 }
 
 void sort_pairs_csr(Ctx &c, uint64_t *keys, uint64_t *keys_alt, int64_t n, int32_t *off,
-                    uint64_t **sorted) {
+                    uint64_t **sorted, uint32_t *vals, uint32_t *vals_alt, uint32_t **vsorted) {
     hipStream_t st = c.stream;
     bool alt = false;
     if (n > 0)
-        alt = radix_sort(keys, keys_alt, nullptr, nullptr, n, 1, 0, 2 * c.sb,
+        alt = radix_sort(keys, keys_alt, vals, vals_alt, n, 1, 0, 2 * c.sb,
                          c.d_rs_scratch.as<uint32_t>(), st);
     uint64_t *r = alt ? keys_alt : keys;
     *sorted = r;
+    if (vsorted) *vsorted = alt ? vals_alt : vals;
     const int64_t nc = c.shape.nc;
     hipLaunchKernelGGL(k_csr, dim3((unsigned)((nc + 1 + 255) / 256)), dim3(256), 0, st, r, n, c.sb,
                        nc, off);
     FMM_LAUNCH_CHECK();
 }
 
+// (t << sb) | s -> (s << sb) | t; with idx, idx[i] = i (the forward position)
 __global__ void k_transpose_keys(const uint64_t *__restrict__ in, int64_t n, int sb,
-                                 uint64_t *__restrict__ out) {
+                                 uint64_t *__restrict__ out, uint32_t *__restrict__ idx) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint64_t k = in[i], mask = (1ull << sb) - 1ull;
     out[i] = ((k & mask) << sb) | (k >> sb);
+    if (idx) idx[i] = (uint32_t)i;
+}
+
+// pos[fwd[e]] = e: the transposed position of every forward pair
+__global__ void k_scatter_pos(const uint32_t *__restrict__ fwd, int64_t n,
+                              uint32_t *__restrict__ pos) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) pos[fwd[e]] = (uint32_t)e;
 }
 
 // Pairs regrouped by source: keys (s << sb) | t sorted, CSR offsets per source.
+// With pos: pos[i] = position in the transposed list of forward pair i.
 void transpose_lists(Ctx &c, const uint64_t *sorted, int64_t n, DevBuf &kT, DevBuf &kT_alt,
-                     DevBuf &offT, uint64_t **sortedT) {
+                     DevBuf &offT, uint64_t **sortedT, DevBuf *pos) {
     kT.ensure(8 * (size_t)std::max<int64_t>(n, 1));
     kT_alt.ensure(8 * (size_t)std::max<int64_t>(n, 1));
     offT.ensure(4 * (size_t)(c.shape.nc + 1));
+    uint32_t *v = nullptr, *va = nullptr, *vs = nullptr;
+    if (pos) {
+        c.d_vals.ensure(4 * (size_t)std::max<int64_t>(n, 1));
+        c.d_vals_alt.ensure(4 * (size_t)std::max<int64_t>(n, 1));
+        pos->ensure(4 * (size_t)std::max<int64_t>(n, 1));
+        v = c.d_vals.as<uint32_t>();
+        va = c.d_vals_alt.as<uint32_t>();
+    }
     if (n > 0) {
         hipLaunchKernelGGL(k_transpose_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                           c.stream, sorted, n, c.sb, kT.as<uint64_t>());
+                           c.stream, sorted, n, c.sb, kT.as<uint64_t>(), v);
         FMM_LAUNCH_CHECK();
     }
-    sort_pairs_csr(c, kT.as<uint64_t>(), kT_alt.as<uint64_t>(), n, offT.as<int32_t>(), sortedT);
+    sort_pairs_csr(c, kT.as<uint64_t>(), kT_alt.as<uint64_t>(), n, offT.as<int32_t>(), sortedT, v,
+                   va, pos ? &vs : nullptr);
+    if (pos && n > 0) {
+        hipLaunchKernelGGL(k_scatter_pos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           c.stream, vs, n, pos->as<uint32_t>());
+        FMM_LAUNCH_CHECK();
+    }
 }
 
 void build_lists_device(Ctx &c, double eta) {
