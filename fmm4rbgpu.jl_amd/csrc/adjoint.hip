@@ -260,14 +260,17 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
 //                 U_c = M_c (forward child multipoles), V = Mbar_P, 4 comps.
 // Geometry: l = sum_c sum_{a,b} (U_c[a].V[b]) Om_c(a,b); dl/dOm_x[ax][bx] =
 // sum_{ay,az} U[a] . (sum_{by,bz} V[b] Om_y[ay][by] Om_z[az][bz]), likewise y, z.
-// PART bit 1: geometry (Xbar); bit 2: the value propagation.
+// PART bit 1: geometry (Xbar); bit 2: the value propagation.  parity >= 0: the
+// launch covers the parents of every level and this one handles those on levels
+// of that parity only (the geometry needs no level order; within one parity no
+// cluster is both a parent and a child, so the Xbar updates do not overlap).
 template <int NP, int MODE, int PART>
 __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__ parents,
                                                      const double *__restrict__ ncoord,
                                                      const double *__restrict__ Uarr,
                                                      const double *__restrict__ Varr,
                                                      double *__restrict__ out,
-                                                     double *__restrict__ Xbar) {
+                                                     double *__restrict__ Xbar, int parity) {
     constexpr int NN = NP * NP * NP, NC = MODE == 0 ? 6 : 4, TPT = (NN + AT - 1) / AT;
     // components handled together per pass (LDS: 4 arrays of CG x NN doubles <= 48 KB)
     constexpr int CG0 = (48 * 1024) / (4 * NN * 8);
@@ -276,10 +279,18 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
     __shared__ double Om[3][NP][NP], dOm[3][NP][NP];  // [axis][child node a][parent node b]
     __shared__ int fl[3][NP];
     __shared__ double pn[3 * NP], cn[3 * NP];
-    __shared__ double bU[CG][NN], bV[CG][NN], t1[CG][NN], t2[CG][NN];
-    __shared__ double red[AT][3 * NP];
+    // staging and contraction buffers; the per-thread partials (red) reuse them
+    // once a child's contractions are done (a barrier separates the two uses)
+    constexpr int BUFW = 4 * CG * NN, REDW = AT * 3 * NP;
+    __shared__ double ubuf[BUFW > REDW ? BUFW : REDW];
+    double(*bU)[NN] = reinterpret_cast<double(*)[NN]>(ubuf);
+    double(*bV)[NN] = reinterpret_cast<double(*)[NN]>(ubuf + CG * NN);
+    double(*t1)[NN] = reinterpret_cast<double(*)[NN]>(ubuf + 2 * CG * NN);
+    double(*t2)[NN] = reinterpret_cast<double(*)[NN]>(ubuf + 3 * CG * NN);
+    double(*red)[3 * NP] = reinterpret_cast<double(*)[3 * NP]>(ubuf);
     const int tid = threadIdx.x;
     const int32_t P = parents[blockIdx.x];
+    if (parity >= 0 && ((31 - __clz(P + 1)) & 1) != parity) return;  // heap level of P
     if (tid < 3 * NP) pn[tid] = ncoord[(int64_t)P * 3 * NP + tid];
     double xbP[3 * NP];  // this thread's share of dl/dX_P
 #pragma unroll
@@ -569,11 +580,12 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
             const T e0 = e[q][0], e1 = e[q][1], e2 = e[q][2];
             const T b0 = b[q][0], b1 = b[q][1], b2 = b[q][2];
             T dxv[NP], dx2v[NP];
-            T Xs[NP];  // column sums / 3 (source x index); row sums in ybuf[NP + y index]
+            T Xs[NP], dx3v[NP];  // column sums / 3 (source x index), 3 dx; row sums in ybuf
 #pragma unroll
             for (int si = 0; si < NP; ++si) {
                 dxv[si] = (T)(tx[q] - sn[si]);
                 dx2v[si] = dxv[si] * dxv[si];
+                dx3v[si] = (T)3 * dxv[si];
                 Xs[si] = 0;
                 if (YS) ybuf[NP + si][tid] = 0.0;
             }
@@ -603,8 +615,15 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                         else y = rsqrt_nr(d);
                         const T y2 = y * y, y3 = y2 * y;
                         const V4 m = smp[si + NP * (sj + NP * sk)];
-                        const T v1 = fma(-dx, b2, v1r), v2 = fma(dx, b1, -v2r);
-                        const T Ru = fma(m.x, fma(dx, e0, er), fma(m.y, v0, fma(m.z, v1, m.w * v2)));
+                        T Ru, u3;  // R.u; u_x / 3 (u = g e + b x m)
+                        if constexpr (YS) {  // R.u = dx u_x + (dy u_y + dz u_z), the latter from row terms
+                            u3 = fma(m.x, u3c[q][0], fma(u3c[q][1], m.w, -(u3c[q][2] * m.z)));
+                            Ru = fma(dx3v[si], u3, fma(m.x, er, fma(m.y, v0, fma(m.z, v1r, -(m.w * v2r)))));
+                        } else {
+                            const T v1 = fma(-dx, b2, v1r), v2 = fma(dx, b1, -v2r);
+                            Ru = fma(m.x, fma(dx, e0, er), fma(m.y, v0, fma(m.z, v1, m.w * v2)));
+                            u3 = 0;
+                        }
                         const T c = y3 * (y2 * Ru);
                         if (si == 0) {
                             G = y3 * m.x;
@@ -628,10 +647,8 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                             Uy = fma(uu, m.z, Uy);
                             Uz = fma(uu, m.w, Uz);
                         }
-                        if constexpr (YS) {  // (J^T u)_x / 3 = y3 u_x / 3 - c (dx + pr px)
-                            const T u3 = fma(m.x, u3c[q][0], fma(u3c[q][1], m.w, -(u3c[q][2] * m.z)));
+                        if constexpr (YS)  // (J^T u)_x / 3 = y3 u_x / 3 - c (dx + pr px)
                             Xs[si] = fma(y3, u3, fma(-c, fma(pr, px, dx), Xs[si]));
-                        }
                     }
                     // pr = px dx + prjk along the row: sum c pr and sum c pr dx from C0..C2
                     const T Cp = fma(px, C1, prjk * C0), Cpx = fma(px, C2, prjk * C1);
@@ -1396,7 +1413,7 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
         const int32_t *par = c.d_lvl_split.as<int32_t>() + b;
 #define ADJ_L2LV(NP)                                                                                \
     hipLaunchKernelGGL((k_adj_transfer<NP, 0, 2>), dim3((unsigned)(e - b)), dim3(AT), 0, st, par, \
-                       ncoord, lbar, loc, lbar, xbar)
+                       ncoord, lbar, loc, lbar, xbar, -1)
         ADJ_NP_SWITCH(c.np, ADJ_L2LV);
         FMM_LAUNCH_CHECK();
     }
@@ -1416,19 +1433,19 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     FMM_LAUNCH_CHECK();
     launch_l2l(c);
     FMM_HIP(hipEventRecord(c.ev[8], st));
-    // 3. geometry of L2P^T and L2L^T (needs the final forward locals)
+    // 3. geometry of L2P^T and L2L^T (needs the final forward locals); the L2L^T
+    //    geometry of all levels in two launches (parents on even, then odd levels)
 #define ADJ_L2PG(NP)                                                                                \
     hipLaunchKernelGGL((k_adj_leaf<NP, 0, 1>), dim3(nleaf), dim3(AT), 0, st, leaves, lo, cnt, sp8,  \
                        ncoord, loc, ob, lbar, gx, gp, xbar)
     ADJ_NP_SWITCH(c.np, ADJ_L2PG);
     FMM_LAUNCH_CHECK();
-    for (int l = S.ml; l >= 1; --l) {
-        const int64_t b = S.split_off[l - 1], e = S.split_off[l];
-        if (e == b) continue;
-        const int32_t *par = c.d_lvl_split.as<int32_t>() + b;
-#define ADJ_L2LG(NP)                                                                                \
-    hipLaunchKernelGGL((k_adj_transfer<NP, 0, 1>), dim3((unsigned)(e - b)), dim3(AT), 0, st, par, \
-                       ncoord, lbar, loc, lbar, xbar)
+    const int64_t npar = S.split_off[S.ml];  // split nodes of levels 0 .. ml-1
+    const int32_t *allpar = c.d_lvl_split.as<int32_t>();
+    for (int parity = 0; parity < 2 && npar > 0; ++parity) {
+#define ADJ_L2LG(NP)                                                                               \
+    hipLaunchKernelGGL((k_adj_transfer<NP, 0, 1>), dim3((unsigned)npar), dim3(AT), 0, st, allpar, \
+                       ncoord, lbar, loc, lbar, xbar, parity)
         ADJ_NP_SWITCH(c.np, ADJ_L2LG);
         FMM_LAUNCH_CHECK();
     }
@@ -1447,15 +1464,23 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     hipLaunchKernelGGL(k_adj_pbar, dim3(1), dim3(256), 0, st, pbar, nc, pbar + 3 * nc);
     FMM_LAUNCH_CHECK();
     FMM_HIP(hipEventRecord(c.ev[9], st));
-    // 5. M2M^T, top-down: parents on level l-1 for l = 1 .. ml
+    // 5. M2M^T values top-down (parents on level l-1 for l = 1 .. ml), then the
+    //    M2M^T geometry of all levels (needs the final Mbar) in two launches
     for (int l = 1; l <= S.ml; ++l) {
         const int64_t b = S.split_off[l - 1], e = S.split_off[l];
         if (e == b) continue;
         const int32_t *par = c.d_lvl_split.as<int32_t>() + b;
 #define ADJ_M2M(NP)                                                                                 \
-    hipLaunchKernelGGL((k_adj_transfer<NP, 1, 3>), dim3((unsigned)(e - b)), dim3(AT), 0, st, par, \
-                       ncoord, c.d_mp4.as<double>(), mbar, mbar, xbar)
+    hipLaunchKernelGGL((k_adj_transfer<NP, 1, 2>), dim3((unsigned)(e - b)), dim3(AT), 0, st, par, \
+                       ncoord, c.d_mp4.as<double>(), mbar, mbar, xbar, -1)
         ADJ_NP_SWITCH(c.np, ADJ_M2M);
+        FMM_LAUNCH_CHECK();
+    }
+    for (int parity = 0; parity < 2 && npar > 0; ++parity) {
+#define ADJ_M2MG(NP)                                                                               \
+    hipLaunchKernelGGL((k_adj_transfer<NP, 1, 1>), dim3((unsigned)npar), dim3(AT), 0, st, allpar, \
+                       ncoord, c.d_mp4.as<double>(), mbar, mbar, xbar, parity)
+        ADJ_NP_SWITCH(c.np, ADJ_M2MG);
         FMM_LAUNCH_CHECK();
     }
     // 6. P2M^T
