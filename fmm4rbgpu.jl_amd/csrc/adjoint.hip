@@ -40,10 +40,10 @@ __device__ inline double abary(int i) {
     return w;
 }
 
-// Weights w and derivatives dw = dw/dx of the NP nodes X at x; returns the
-// singular node (last within eps) or -1.
+// Weights w and derivatives dw = dw/dx of the NP nodes X at x, and ri = 1/(x - X)
+// (0 in the singular case); returns the singular node (last within eps) or -1.
 template <int NP>
-__device__ inline int lgw_d(double x, const double *X, double *w, double *dw) {
+__device__ inline int lgw_d(double x, const double *X, double *w, double *dw, double *ri) {
     int flag = -1;
 #pragma unroll
     for (int i = 0; i < NP; ++i)
@@ -55,6 +55,7 @@ __device__ inline int lgw_d(double x, const double *X, double *w, double *dw) {
         for (int i = 0; i < NP; ++i) {
             w[i] = (i == flag) ? 1.0 : 0.0;
             dw[i] = 0.0;
+            ri[i] = 0.0;
             if (i != flag) {
                 dw[i] = (abary<NP>(i) / bf) / (X[flag] - X[i]);
                 s += dw[i];
@@ -66,29 +67,31 @@ __device__ inline int lgw_d(double x, const double *X, double *w, double *dw) {
     double S = 0.0;
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
-        w[i] = abary<NP>(i) / (x - X[i]);
+        ri[i] = 1.0 / (x - X[i]);
+        w[i] = abary<NP>(i) * ri[i];  // abary = +-1 or +-1/2: exact
         S += w[i];
     }
+    const double iS = 1.0 / S;
     double t = 0.0;
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
-        w[i] = w[i] / S;
-        t += w[i] / (x - X[i]);
+        w[i] *= iS;
+        t = fma(w[i], ri[i], t);
     }
 #pragma unroll
-    for (int i = 0; i < NP; ++i) dw[i] = w[i] * (t - 1.0 / (x - X[i]));
+    for (int i = 0; i < NP; ++i) dw[i] = w[i] * (t - ri[i]);
     return -1;
 }
 
 // Given l = sum_a c_a w_a(x; X): returns dl/dx and adds dl/dX_m to Xb[m].
 template <int NP>
-__device__ inline double axis_adj(double x, const double *X, const double *w, const double *dw,
-                                  int flag, const double *c, double *Xb) {
+__device__ inline double axis_adj(const double *w, const double *dw, const double *ri, int flag,
+                                  const double *c, double *Xb) {
     double xb = 0.0, cw = 0.0;
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
-        xb += c[i] * dw[i];
-        cw += c[i] * w[i];
+        xb = fma(c[i], dw[i], xb);
+        cw = fma(c[i], w[i], cw);
     }
     if (flag >= 0) {
 #pragma unroll
@@ -96,7 +99,7 @@ __device__ inline double axis_adj(double x, const double *X, const double *w, co
             if (m == flag) Xb[m] -= xb;
     } else {
 #pragma unroll
-        for (int m = 0; m < NP; ++m) Xb[m] += w[m] / (x - X[m]) * (c[m] - cw);
+        for (int m = 0; m < NP; ++m) Xb[m] = fma(w[m] * ri[m], c[m] - cw, Xb[m]);
     }
     return xb;
 }
@@ -157,9 +160,9 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
     for (int p = tid; (PART & 1) && p < c; p += AT) {
         const int64_t i = a + p;
         const double *x = sp8 + 8 * i;
-        double w[3][NP], dw[3][NP];
+        double w[3][NP], dw[3][NP], ri[3][NP];
         int fl[3];
-        for (int d = 0; d < 3; ++d) fl[d] = lgw_d<NP>(x[d], nd + d * NP, w[d], dw[d]);
+        for (int d = 0; d < 3; ++d) fl[d] = lgw_d<NP>(x[d], nd + d * NP, w[d], dw[d], ri[d]);
         double o[NC];
         if (MODE == 0) {
 #pragma unroll
@@ -178,28 +181,37 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
         double qb[NC];
 #pragma unroll
         for (int m = 0; m < NC; ++m) qb[m] = 0.0;
+        // T[d][m] = sum over the nodes with index m on axis d of s W / w_d[m], s = o.A[node]:
+        // T0 per node, T1 and T2 from the row sums R1 = sum_ix s w0[ix] (likewise qb)
+#pragma unroll 1
         for (int kz = 0; kz < NP; ++kz)
+#pragma unroll 1
             for (int jy = 0; jy < NP; ++jy) {
                 const double wyz = w[1][jy] * w[2][kz];
+                double R1 = 0.0, Q[NC];
+#pragma unroll
+                for (int m = 0; m < NC; ++m) Q[m] = 0.0;
 #pragma unroll
                 for (int ix = 0; ix < NP; ++ix) {
                     const double *Av = Ak + (ix + NP * (jy + NP * kz)) * NC;
-                    double s = 0.0;
+                    double s = o[0] * Av[0];
 #pragma unroll
-                    for (int m = 0; m < NC; ++m) s += o[m] * Av[m];
-                    T[0][ix] += s * wyz;
-                    T[1][jy] += s * (w[0][ix] * w[2][kz]);
-                    T[2][kz] += s * (w[0][ix] * w[1][jy]);
-                    if (MODE == 1) {
-                        const double W3 = w[0][ix] * wyz;
+                    for (int m = 1; m < NC; ++m) s = fma(o[m], Av[m], s);
+                    T[0][ix] = fma(s, wyz, T[0][ix]);
+                    R1 = fma(s, w[0][ix], R1);
+                    if (MODE == 1)
 #pragma unroll
-                        for (int m = 0; m < NC; ++m) qb[m] += Av[m] * W3;
-                    }
+                        for (int m = 0; m < NC; ++m) Q[m] = fma(w[0][ix], Av[m], Q[m]);
                 }
+                T[1][jy] = fma(R1, w[2][kz], T[1][jy]);
+                T[2][kz] = fma(R1, w[1][jy], T[2][kz]);
+                if (MODE == 1)
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) qb[m] = fma(Q[m], wyz, qb[m]);
             }
         double g[3];
         for (int d = 0; d < 3; ++d)
-            g[d] = axis_adj<NP>(x[d], nd + d * NP, w[d], dw[d], fl[d], T[d], xbl + d * NP);
+            g[d] = axis_adj<NP>(w[d], dw[d], ri[d], fl[d], T[d], xbl + d * NP);
         for (int d = 0; d < 3; ++d) gx[3 * i + d] += g[d];
         if (MODE == 1) {
             const double gi = qb[0] / x[3];
@@ -223,8 +235,8 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
             if (p < c) {
                 const int64_t i = a + p;
                 const double *x = sp8 + 8 * i;
-                double dw[NP];
-                for (int d = 0; d < 3; ++d) lgw_d<NP>(x[d], nd + d * NP, &Wt[tid][d * NP], dw);
+                double dw[NP], ri[NP];
+                for (int d = 0; d < 3; ++d) lgw_d<NP>(x[d], nd + d * NP, &Wt[tid][d * NP], dw, ri);
                 for (int m = 0; m < 6; ++m) Ot[tid][m] = ob[6 * i + m];
             }
             __syncthreads();
@@ -237,7 +249,7 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
                     for (int pp = 0; pp < mm; ++pp) {
                         const double W3 = (Wt[pp][ix] * Wt[pp][NP + jy]) * Wt[pp][2 * NP + kz];
 #pragma unroll
-                        for (int m = 0; m < 6; ++m) acc[q][m] += Ot[pp][m] * W3;
+                        for (int m = 0; m < 6; ++m) acc[q][m] = fma(Ot[pp][m], W3, acc[q][m]);
                     }
                 }
             }
@@ -307,7 +319,8 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
         __syncthreads();
         if (tid < 3 * NP) {
             const int d = tid / NP, ai = tid % NP;
-            fl[d][ai] = lgw_d<NP>(cn[tid], pn + d * NP, &Om[d][ai][0], &dOm[d][ai][0]);
+            double ri[NP];
+            fl[d][ai] = lgw_d<NP>(cn[tid], pn + d * NP, &Om[d][ai][0], &dOm[d][ai][0], ri);
         }
         // dl/dOm accumulated by the thread owning the (a_d, b_d) pair
         double gO[3] = {0.0, 0.0, 0.0};
@@ -459,8 +472,10 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
             for (int b = 0; b < NP; ++b) cvec[b] = red[ai * NP + b][d];
             double xP[NP];
             for (int m = 0; m < NP; ++m) xP[m] = 0.0;
-            const double gxc = axis_adj<NP>(cn[tid], pn + d * NP, &Om[d][ai][0], &dOm[d][ai][0],
-                                            fl[d][ai], cvec, xP);
+            double ri[NP];  // 1/(x - X) for the node-coordinate terms
+            if (fl[d][ai] < 0)
+                for (int m = 0; m < NP; ++m) ri[m] = 1.0 / (cn[tid] - pn[d * NP + m]);
+            const double gxc = axis_adj<NP>(&Om[d][ai][0], &dOm[d][ai][0], ri, fl[d][ai], cvec, xP);
             Xbar[(int64_t)C * 3 * NP + tid] += gxc;
             for (int m = 0; m < NP; ++m) xbP[d * NP + m] += xP[m];
         }
@@ -1257,7 +1272,8 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
 __global__ void k_adj_prep(int64_t N, const int32_t *__restrict__ parind,
                            const double *__restrict__ Ebar, const double *__restrict__ Bbar,
                            double amp, double *__restrict__ ob, double *__restrict__ gx,
-                           double *__restrict__ gp) {
+                           double *__restrict__ gp, double *__restrict__ gx2,
+                           double *__restrict__ gp2) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int64_t q = parind[i];
@@ -1266,6 +1282,8 @@ __global__ void k_adj_prep(int64_t N, const int32_t *__restrict__ parind,
         ob[6 * i + 3 + d] = amp * Bbar[3 * q + d];
         gx[3 * i + d] = 0.0;
         gp[3 * i + d] = 0.0;
+        gx2[3 * i + d] = 0.0;
+        gp2[3 * i + d] = 0.0;
     }
 }
 
@@ -1300,11 +1318,13 @@ __global__ void k_adj_finish(int64_t N, const int32_t *__restrict__ parind,
                              const int32_t *__restrict__ leafof, const int32_t *__restrict__ ext,
                              const double *__restrict__ abar, const double *__restrict__ pbar,
                              const double *__restrict__ gx, const double *__restrict__ gp,
+                             const double *__restrict__ gx2, const double *__restrict__ gp2,
                              double *__restrict__ gpos, double *__restrict__ gmom) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int32_t p = parind[i];
-    double g[3] = {gx[3 * i], gx[3 * i + 1], gx[3 * i + 2]};
+    double g[3] = {gx[3 * i] + gx2[3 * i], gx[3 * i + 1] + gx2[3 * i + 1],
+                   gx[3 * i + 2] + gx2[3 * i + 2]};
     int32_t k = leafof[i];
     for (;;) {
         for (int qd = 0; qd < 6; ++qd)
@@ -1314,7 +1334,7 @@ __global__ void k_adj_finish(int64_t N, const int32_t *__restrict__ parind,
     }
     for (int d = 0; d < 3; ++d) {
         gpos[3 * (int64_t)p + d] = g[d];
-        gmom[3 * (int64_t)p + d] = gp[3 * i + d] + pbar[d] / (double)N;
+        gmom[3 * (int64_t)p + d] = (gp[3 * i + d] + gp2[3 * i + d]) + pbar[d] / (double)N;
     }
 }
 
@@ -1363,6 +1383,8 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     c.d_ob.ensure(6 * (size_t)N * 8);
     c.d_gx.ensure(3 * (size_t)N * 8);
     c.d_gp.ensure(3 * (size_t)N * 8);
+    c.d_gx2.ensure(3 * (size_t)N * 8);
+    c.d_gp2.ensure(3 * (size_t)N * 8);
     c.d_lbar.ensure(6 * (size_t)NN * nc * 8);
     c.d_mbar.ensure(4 * (size_t)NN * nc * 8);
     c.d_xbar.ensure(3 * (size_t)np * nc * 8);
@@ -1383,20 +1405,51 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     double *ysp = ysplit ? c.d_ysp.as<double>() : nullptr;
 
     double *ob = c.d_ob.as<double>(), *gx = c.d_gx.as<double>(), *gp = c.d_gp.as<double>();
+    double *gx2 = c.d_gx2.as<double>(), *gp2 = c.d_gp2.as<double>();
     double *lbar = c.d_lbar.as<double>(), *mbar = c.d_mbar.as<double>();
     double *xbar = c.d_xbar.as<double>();
     hipLaunchKernelGGL(k_adj_prep, dim3(nbk(N, 256)), dim3(256), 0, st, N,
-                       c.d_parind.as<int32_t>(), Ebar, Bbar, amp, ob, gx, gp);
-    FMM_LAUNCH_CHECK();
-    const int64_t nx = 3 * np * nc;
-    hipLaunchKernelGGL(k_fill0, dim3(nbk(nx, 256)), dim3(256), 0, st, xbar, nx);
+                       c.d_parind.as<int32_t>(), Ebar, Bbar, amp, ob, gx, gp, gx2, gp2);
     FMM_LAUNCH_CHECK();
     const int32_t *leaves = c.d_leaves.as<int32_t>(), *lo = c.d_lo.as<int32_t>(),
                   *cnt = c.d_cnt.as<int32_t>();
     const double *sp8 = c.d_sp8.as<double>(), *ncoord = c.d_ncoord.as<double>();
+    const bool mixed = c.precision == FMM4RB_MIXED;
+    // P2P^T needs only the particles, the cotangents and the near-field lists: it
+    // runs on the auxiliary stream into gx2/gp2 while this stream does the far field
+    {
+        hipStream_t sa = c.aux;
+        FMM_HIP(hipEventRecord(c.evx[0], st));
+        FMM_HIP(hipStreamWaitEvent(sa, c.evx[0], 0));
+        FMM_HIP(hipEventRecord(c.evx[1], sa));
+        int bt = ((S.maxleaf + 63) / 64) * 64;
+        bt = std::max(64, std::min(bt, PT));
+        const size_t v4 = mixed ? sizeof(float4) : sizeof(double4);
+        const size_t lds_t = (size_t)bt * (sizeof(double4) + v4);
+        const size_t lds_s = (size_t)bt * (sizeof(double4) + 2 * v4);
+        if (mixed) {
+            hipLaunchKernelGGL(k_adj_p2p_t<true>, dim3(nleaf), dim3(bt), lds_t, sa, leaves,
+                               c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, lo, cnt, sp8, ob, gx2);
+            FMM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_adj_p2p_s<true>, dim3(nleaf), dim3(bt), lds_s, sa, leaves,
+                               c.d_p2pT_off.as<int32_t>(), c.p2pT_sorted, c.sb, lo, cnt, sp8, ob,
+                               gx2, gp2);
+        } else {
+            hipLaunchKernelGGL(k_adj_p2p_t<false>, dim3(nleaf), dim3(bt), lds_t, sa, leaves,
+                               c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, lo, cnt, sp8, ob, gx2);
+            FMM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_adj_p2p_s<false>, dim3(nleaf), dim3(bt), lds_s, sa, leaves,
+                               c.d_p2pT_off.as<int32_t>(), c.p2pT_sorted, c.sb, lo, cnt, sp8, ob,
+                               gx2, gp2);
+        }
+        FMM_LAUNCH_CHECK();
+        FMM_HIP(hipEventRecord(c.evx[2], sa));
+    }
+    const int64_t nx = 3 * np * nc;
+    hipLaunchKernelGGL(k_fill0, dim3(nbk(nx, 256)), dim3(256), 0, st, xbar, nx);
+    FMM_LAUNCH_CHECK();
     double *loc = c.d_loc.as<double>(), *pbar = c.d_pbar.as<double>();
     // mixed precision: the M2L^T kernels read the float4 copy of the multipoles
-    const bool mixed = c.precision == FMM4RB_MIXED;
     if (mixed) launch_mp_to_f32(c);
     const void *mpv = mixed ? (const void *)c.d_mp4f.p : (const void *)c.d_mp4.p;
 
@@ -1489,36 +1542,15 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
                        ncoord, mbar, ob, (double *)nullptr, gx, gp, xbar)
     ADJ_NP_SWITCH(c.np, ADJ_P2M);
     FMM_LAUNCH_CHECK();
-    // 7. P2P^T
-    int bt = ((S.maxleaf + 63) / 64) * 64;
-    bt = std::max(64, std::min(bt, PT));
-    const size_t v4 = mixed ? sizeof(float4) : sizeof(double4);
-    const size_t lds_t = (size_t)bt * (sizeof(double4) + v4);
-    const size_t lds_s = (size_t)bt * (sizeof(double4) + 2 * v4);
-    if (mixed) {
-        hipLaunchKernelGGL(k_adj_p2p_t<true>, dim3(nleaf), dim3(bt), lds_t, st, leaves,
-                           c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, lo, cnt, sp8, ob, gx);
-        FMM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_adj_p2p_s<true>, dim3(nleaf), dim3(bt), lds_s, st, leaves,
-                           c.d_p2pT_off.as<int32_t>(), c.p2pT_sorted, c.sb, lo, cnt, sp8, ob, gx,
-                           gp);
-    } else {
-        hipLaunchKernelGGL(k_adj_p2p_t<false>, dim3(nleaf), dim3(bt), lds_t, st, leaves,
-                           c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, lo, cnt, sp8, ob, gx);
-        FMM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_adj_p2p_s<false>, dim3(nleaf), dim3(bt), lds_s, st, leaves,
-                           c.d_p2pT_off.as<int32_t>(), c.p2pT_sorted, c.sb, lo, cnt, sp8, ob, gx,
-                           gp);
-    }
-    FMM_LAUNCH_CHECK();
     // bounding boxes -> extreme particles, mean momentum, output order
     hipLaunchKernelGGL(k_adj_bbox, dim3(nbk(nc, 128)), dim3(128), 0, st, nc, c.np,
                        c.d_flags.as<uint8_t>(), xbar, c.d_costab.as<double>(),
                        c.d_abar.as<double>());
     FMM_LAUNCH_CHECK();
+    FMM_HIP(hipStreamWaitEvent(st, c.evx[2], 0));  // join the P2P^T stream
     hipLaunchKernelGGL(k_adj_finish, dim3(nbk(N, 256)), dim3(256), 0, st, N,
                        c.d_parind.as<int32_t>(), c.d_leafof.as<int32_t>(), c.d_ext.as<int32_t>(),
-                       c.d_abar.as<double>(), pbar + 3 * nc, gx, gp, gpos, gmom);
+                       c.d_abar.as<double>(), pbar + 3 * nc, gx, gp, gx2, gp2, gpos, gmom);
     FMM_LAUNCH_CHECK();
 }
 

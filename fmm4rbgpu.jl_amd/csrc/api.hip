@@ -2,9 +2,11 @@
 // shape, device buffers and the field-evaluation pipeline.
 //
 // Pipeline (update_particles_field!(beam, FMMGPU(n, N0, eta); lambda),
-// src/update_particles_field.jl:66-148), all on one stream:
-//   p_avg + input check -> cluster tree -> node geometry -> P2M, M2M
-//   -> dual traversal, list sort/CSR -> M2L -> P2P -> L2L -> L2P + scaling.
+// src/update_particles_field.jl:66-148):
+//   p_avg + input check -> cluster tree -> node geometry -> dual traversal,
+//   list sort/CSR -> { P2M, M2M -> M2L -> L2L  ||  P2P } -> L2P + scaling,
+// where P2P runs on the context's second stream, concurrently with the
+// far-field chain (whose per-level launches leave most of the GPU idle).
 // Unlike the reference, the tree and the lists are built on the device and
 // nothing but a handful of counters crosses PCIe between stages.
 #include <algorithm>
@@ -244,10 +246,17 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
     build_tree_device(c, pos, mom, N, N0);
     launch_node_geo(c);
     FMM_HIP(hipEventRecord(c.ev[2], st));
+    build_lists_device(c, eta);
+    FMM_HIP(hipEventRecord(c.ev[3], st));
+    // the near field needs only the tree order and the lists: it runs on the
+    // auxiliary stream while this one forms the far field, and L2P waits for it
+    FMM_HIP(hipEventRecord(c.evx[0], st));
+    FMM_HIP(hipStreamWaitEvent(c.aux, c.evx[0], 0));
+    FMM_HIP(hipEventRecord(c.evx[1], c.aux));
+    launch_p2p(c, c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, c.aux);
+    FMM_HIP(hipEventRecord(c.evx[2], c.aux));
     launch_upward(c);
     if (precision == FMM4RB_MIXED) launch_mp_to_f32(c);
-    FMM_HIP(hipEventRecord(c.ev[3], st));
-    build_lists_device(c, eta);
     FMM_HIP(hipEventRecord(c.ev[4], st));
     launch_m2l(c, c.d_m2l_off.as<int32_t>(), c.m2l_sorted, c.sb, c.shape.nc);
     FMM_HIP(hipEventRecord(c.ev[5], st));
@@ -257,9 +266,9 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
                                hipMemcpyDeviceToHost, st));
         FMM_HIP(hipStreamSynchronize(st));
     }
-    launch_p2p(c, c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb);
-    FMM_HIP(hipEventRecord(c.ev[6], st));
     launch_l2l(c);
+    FMM_HIP(hipStreamWaitEvent(st, c.evx[2], 0));
+    FMM_HIP(hipEventRecord(c.ev[6], st));
     launch_l2p(c, amp, E, B);
     FMM_HIP(hipEventRecord(c.ev[7], st));
     unsigned long long ls[2] = {0, 0};
@@ -294,10 +303,14 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
         stats->p2p_particle_pairs = (int64_t)ls[0];
         for (int d = 0; d < 3; ++d) stats->p_avg[d] = c.pavg_host[d];
         stats->g_avg = c.pavg_host[3];
-        float *t[8] = {&stats->ms_total, &stats->ms_pavg, &stats->ms_tree, &stats->ms_upward,
-                       &stats->ms_lists, &stats->ms_m2l, &stats->ms_p2p, &stats->ms_downward};
-        FMM_HIP(hipEventElapsedTime(t[0], c.ev[0], c.ev[7]));
-        for (int s = 1; s < 8; ++s) FMM_HIP(hipEventElapsedTime(t[s], c.ev[s - 1], c.ev[s]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_total, c.ev[0], c.ev[7]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_pavg, c.ev[0], c.ev[1]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_tree, c.ev[1], c.ev[2]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_lists, c.ev[2], c.ev[3]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_upward, c.ev[3], c.ev[4]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_m2l, c.ev[4], c.ev[5]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_p2p, c.evx[1], c.evx[2]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_downward, c.ev[5], c.ev[7]));
     }
 }
 
@@ -346,8 +359,8 @@ static void run_vjp(Ctx &c, int64_t N, const double *pos, const double *mom, dou
         FMM_HIP(hipEventElapsedTime(&stats->ms_upward, c.ev[2], c.ev[3]));
         FMM_HIP(hipEventElapsedTime(&stats->ms_lists, c.ev[3], c.ev[4]));
         FMM_HIP(hipEventElapsedTime(&stats->ms_m2l, c.ev[5], c.ev[8]));
-        FMM_HIP(hipEventElapsedTime(&stats->ms_p2p, c.ev[8], c.ev[9]));
-        FMM_HIP(hipEventElapsedTime(&stats->ms_downward, c.ev[9], c.ev[6]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_p2p, c.evx[1], c.evx[2]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_downward, c.ev[8], c.ev[6]));
     }
 }
 
@@ -396,8 +409,10 @@ int fmm4rb_create(fmm4rb_ctx **out, int device) {
     ctx->c.device = device;
     try {
         FMM_HIP(hipStreamCreateWithFlags(&ctx->c.own, hipStreamNonBlocking));
+        FMM_HIP(hipStreamCreateWithFlags(&ctx->c.aux, hipStreamNonBlocking));
         ctx->c.stream = ctx->c.own;
         for (auto &e : ctx->c.ev) FMM_HIP(hipEventCreate(&e));
+        for (auto &e : ctx->c.evx) FMM_HIP(hipEventCreate(&e));
     } catch (...) {
         delete ctx;
         throw;
@@ -411,13 +426,15 @@ int fmm4rb_destroy(fmm4rb_ctx *ctx) {
     if (!ctx) return FMM4RB_OK;
     (void)hipSetDevice(ctx->c.device);
     if (ctx->c.own) (void)hipStreamSynchronize(ctx->c.own);
+    if (ctx->c.aux) (void)hipStreamSynchronize(ctx->c.aux);
     Ctx &c = ctx->c;
     DevBuf *bufs[] = {&c.d_lo, &c.d_cnt, &c.d_flags, &c.d_leaves, &c.d_lvl_nodes, &c.d_lvl_split,
                       &c.d_leftbase, &c.d_part, &c.d_pavg, &c.d_bad, &c.d_keys, &c.d_keys_alt,
                       &c.d_vals, &c.d_vals_alt, &c.d_rs_scratch, &c.d_seg, &c.d_seg_alt,
                       &c.d_rk, &c.d_pivot, &c.d_bsum, &c.d_axis, &c.d_parind, &c.d_leafof,
                       &c.d_bbox, &c.d_geo, &c.d_ncoord, &c.d_costab, &c.d_ext, &c.d_sp8, &c.d_sp8f,
-                      &c.d_ob, &c.d_gx, &c.d_gp, &c.d_lbar, &c.d_mbar, &c.d_xbar, &c.d_pbar,
+                      &c.d_ob, &c.d_gx, &c.d_gp, &c.d_gx2, &c.d_gp2, &c.d_lbar, &c.d_mbar,
+                      &c.d_xbar, &c.d_pbar,
                       &c.d_abar, &c.d_p2pT, &c.d_p2pT_alt, &c.d_m2lT, &c.d_m2lT_alt,
                       &c.d_p2pT_off, &c.d_m2lT_off,
                       &c.d_front0, &c.d_front1, &c.d_tcnt, &c.d_p2p, &c.d_p2p_alt, &c.d_m2l,
@@ -427,7 +444,10 @@ int fmm4rb_destroy(fmm4rb_ctx *ctx) {
     for (DevBuf *b : bufs) b->release();
     for (auto &e : c.ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : c.evx)
+        if (e) (void)hipEventDestroy(e);
     if (c.own) (void)hipStreamDestroy(c.own);
+    if (c.aux) (void)hipStreamDestroy(c.aux);
     delete ctx;
     FMM_API_END
 }
@@ -826,7 +846,7 @@ int fmm4rb_op_p2p(fmm4rb_ctx *ctx, int precision, int64_t N, const double *pos, 
     uint64_t *sorted = nullptr;
     hook_pairs(c, nc, npairs, pairs, c.d_p2p, c.d_p2p_alt, c.d_p2p_off, &sorted);
     c.cap_p2p = std::max<int64_t>(c.cap_p2p, (int64_t)(c.d_p2p.cap / 8));
-    if (!S.leaves.empty()) launch_p2p(c, c.d_p2p_off.as<int32_t>(), sorted, c.sb);
+    if (!S.leaves.empty()) launch_p2p(c, c.d_p2p_off.as<int32_t>(), sorted, c.sb, c.stream);
     std::vector<double> nf(6 * (size_t)N);
     FMM_HIP(hipMemcpyAsync(nf.data(), c.d_near.p, 8 * nf.size(), hipMemcpyDeviceToHost, st));
     FMM_HIP(hipStreamSynchronize(st));

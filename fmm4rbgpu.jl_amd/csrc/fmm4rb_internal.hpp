@@ -129,7 +129,7 @@ void launch_mp_to_f32(Ctx &c);
 
 // ---- interactions (m2l.hip, p2p.hip)
 void launch_m2l(Ctx &c, const int32_t *off, const uint64_t *keys, int sb, int64_t ngroups_grid);
-void launch_p2p(Ctx &c, const int32_t *off, const uint64_t *keys, int sb);
+void launch_p2p(Ctx &c, const int32_t *off, const uint64_t *keys, int sb, hipStream_t st);
 
 // ---- adjoint (adjoint.hip, traverse.hip)
 void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, double *gpos,
@@ -146,7 +146,10 @@ void launch_bruteforce(const double *pos, const double *mom, int64_t N, const in
 struct Ctx {
     int device = 0;
     hipStream_t own = nullptr, stream = nullptr;
-    hipEvent_t ev[10] = {};
+    // the near field (P2P, P2P^T) runs on aux, concurrently with the far-field chain:
+    // evx[0] fork, evx[1] P2P start, evx[2] P2P end (the join)
+    hipStream_t aux = nullptr;
+    hipEvent_t ev[10] = {}, evx[3] = {};
     bool debug = false;
 
     // parameters of the current call
@@ -175,6 +178,7 @@ struct Ctx {
     DevBuf d_mp4, d_mp4f, d_loc, d_near;
     // adjoint (field_vjp)
     DevBuf d_ob, d_gx, d_gp, d_lbar, d_mbar, d_xbar, d_pbar, d_abar;
+    DevBuf d_gx2, d_gp2;  // P2P^T gradients (written on aux)
     DevBuf d_p2pT, d_p2pT_alt, d_m2lT, d_m2lT_alt, d_p2pT_off, d_m2lT_off;
     DevBuf d_m2lT_pos, d_ysp;  // transposed position of each M2L pair; per-pair source sums
     uint64_t *p2pT_sorted = nullptr, *m2lT_sorted = nullptr;

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
     }
 }
 
-void launch_p2p(Ctx &c, const int32_t *off, const uint64_t *keys, int sb) {
+void launch_p2p(Ctx &c, const int32_t *off, const uint64_t *keys, int sb, hipStream_t st) {
     const unsigned nleaf = (unsigned)c.shape.leaves.size();
     const bool mixed = c.precision == FMM4RB_MIXED;
     const float *q4 = mixed ? c.d_sp8f.as<float>() : (const float *)nullptr;
@@ -195,7 +195,7 @@ void launch_p2p(Ctx &c, const int32_t *off, const uint64_t *keys, int sb) {
     const size_t lds = (size_t)bt * sizeof(double2) + (size_t)((bt + 1) / 2) * sizeof(double2) +
                        (size_t)bt * (mixed ? sizeof(float4) : sizeof(double4));
 #define P2P_L(MX, TP)                                                                            \
-    hipLaunchKernelGGL((k_p2p<MX, TP>), dim3(nleaf), dim3(bt), lds, c.stream,                    \
+    hipLaunchKernelGGL((k_p2p<MX, TP>), dim3(nleaf), dim3(bt), lds, st,                          \
                        c.d_leaves.as<int32_t>(), off, keys, sb, c.d_lo.as<int32_t>(),             \
                        c.d_cnt.as<int32_t>(), c.d_sp8.as<double>(), q4, c.d_near.as<double>())
     if (mixed) {

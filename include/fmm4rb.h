@@ -36,7 +36,10 @@ extern "C" {
 
 typedef struct fmm4rb_ctx fmm4rb_ctx;
 
-/* Per-call counters and stage times (HIP events on the context stream). */
+/* Per-call counters and stage times (HIP events).  The near field (P2P, and
+ * P2P^T in reverse mode) runs on a second stream of the context, concurrently
+ * with the far-field stages: ms_p2p is its own duration, ms_downward includes
+ * waiting for it, and ms_total is the wall time of the whole call. */
 typedef struct {
     int64_t N, N0, ncluster, nleaf;
     int32_t n, precision, maxlevel, traversal_iterations;
@@ -52,6 +55,8 @@ const char *fmm4rb_version(void);
 const char *fmm4rb_last_error(void);
 
 /* No reference counterpart: the reference uses CUDA.jl's implicit context. */
+/* A context owns two HIP streams (default priority): its own, and one for the
+ * near field, forked from and joined back into the stream the call runs on. */
 int fmm4rb_create(fmm4rb_ctx **ctx, int device);
 int fmm4rb_destroy(fmm4rb_ctx *ctx);
 /* Run the context's work on a caller stream (hipStream_t; NULL restores the
@@ -88,8 +93,8 @@ int fmm4rb_field_device(fmm4rb_ctx *ctx, int64_t N, const double *d_pos, const d
  * interpolation-weight and bounding-box terms stay FP64.  stats: ms_m2l =
  * transposed lists, local cotangents and the
  * forward M2L fused with the M2L^T target side plus the forward L2L; ms_p2p =
- * L2P^T/L2L^T geometry and the M2L^T source side; ms_downward = M2M^T, P2M^T,
- * P2P^T and the gradient assembly.
+ * P2P^T (both sides, on the near-field stream); ms_downward = L2P^T/L2L^T
+ * geometry, the M2L^T source side, M2M^T, P2M^T and the gradient assembly.
  */
 int fmm4rb_field_vjp(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
                      double charge, double lambda, int n, int64_t N0, double eta, int precision,
