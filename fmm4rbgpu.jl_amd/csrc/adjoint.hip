@@ -160,9 +160,12 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
     for (int p = tid; (PART & 1) && p < c; p += AT) {
         const int64_t i = a + p;
         const double *x = sp8 + 8 * i;
-        double w[3][NP], dw[3][NP], ri[3][NP];
+        double w[3][NP], dw[3][NP];
         int fl[3];
-        for (int d = 0; d < 3; ++d) fl[d] = lgw_d<NP>(x[d], nd + d * NP, w[d], dw[d], ri[d]);
+        for (int d = 0; d < 3; ++d) {
+            double ri[NP];  // not kept across the node loop (registers): recomputed below
+            fl[d] = lgw_d<NP>(x[d], nd + d * NP, w[d], dw[d], ri);
+        }
         double o[NC];
         if (MODE == 0) {
 #pragma unroll
@@ -182,36 +185,46 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
 #pragma unroll
         for (int m = 0; m < NC; ++m) qb[m] = 0.0;
         // T[d][m] = sum over the nodes with index m on axis d of s W / w_d[m], s = o.A[node]:
-        // T0 per node, T1 and T2 from the row sums R1 = sum_ix s w0[ix] (likewise qb)
-#pragma unroll 1
-        for (int kz = 0; kz < NP; ++kz)
-#pragma unroll 1
-            for (int jy = 0; jy < NP; ++jy) {
-                const double wyz = w[1][jy] * w[2][kz];
-                double R1 = 0.0, Q[NC];
+        // T0 per node, T1 and T2 from the row sums R1 = sum_ix s w0[ix]; qb per node
+        auto row = [&](int kz, int jy) {
+            const double wyz = w[1][jy] * w[2][kz];
+            double R1 = 0.0;
 #pragma unroll
-                for (int m = 0; m < NC; ++m) Q[m] = 0.0;
+            for (int ix = 0; ix < NP; ++ix) {
+                const double *Av = Ak + (ix + NP * (jy + NP * kz)) * NC;
+                double s = o[0] * Av[0];
 #pragma unroll
-                for (int ix = 0; ix < NP; ++ix) {
-                    const double *Av = Ak + (ix + NP * (jy + NP * kz)) * NC;
-                    double s = o[0] * Av[0];
+                for (int m = 1; m < NC; ++m) s = fma(o[m], Av[m], s);
+                T[0][ix] = fma(s, wyz, T[0][ix]);
+                R1 = fma(s, w[0][ix], R1);
+                if (MODE == 1) {
+                    const double W3 = w[0][ix] * wyz;
 #pragma unroll
-                    for (int m = 1; m < NC; ++m) s = fma(o[m], Av[m], s);
-                    T[0][ix] = fma(s, wyz, T[0][ix]);
-                    R1 = fma(s, w[0][ix], R1);
-                    if (MODE == 1)
-#pragma unroll
-                        for (int m = 0; m < NC; ++m) Q[m] = fma(w[0][ix], Av[m], Q[m]);
+                    for (int m = 0; m < NC; ++m) qb[m] = fma(Av[m], W3, qb[m]);
                 }
-                T[1][jy] = fma(R1, w[2][kz], T[1][jy]);
-                T[2][kz] = fma(R1, w[1][jy], T[2][kz]);
-                if (MODE == 1)
-#pragma unroll
-                    for (int m = 0; m < NC; ++m) qb[m] = fma(Q[m], wyz, qb[m]);
             }
+            T[1][jy] = fma(R1, w[2][kz], T[1][jy]);
+            T[2][kz] = fma(R1, w[1][jy], T[2][kz]);
+        };
+        if constexpr (MODE == 0) {  // rolled: 6 components unrolled over the rows spill
+#pragma unroll 1
+            for (int kz = 0; kz < NP; ++kz)
+#pragma unroll 1
+                for (int jy = 0; jy < NP; ++jy) row(kz, jy);
+        } else {  // planes rolled, rows unrolled
+#pragma unroll 1
+            for (int kz = 0; kz < NP; ++kz)
+#pragma unroll
+                for (int jy = 0; jy < NP; ++jy) row(kz, jy);
+        }
         double g[3];
-        for (int d = 0; d < 3; ++d)
-            g[d] = axis_adj<NP>(w[d], dw[d], ri[d], fl[d], T[d], xbl + d * NP);
+        for (int d = 0; d < 3; ++d) {
+            double ri[NP];
+            if (fl[d] < 0)
+#pragma unroll
+                for (int m = 0; m < NP; ++m) ri[m] = 1.0 / (x[d] - nd[d * NP + m]);
+            g[d] = axis_adj<NP>(w[d], dw[d], ri, fl[d], T[d], xbl + d * NP);
+        }
         for (int d = 0; d < 3; ++d) gx[3 * i + d] += g[d];
         if (MODE == 1) {
             const double gi = qb[0] / x[3];
@@ -533,7 +546,8 @@ struct YSplit {
 };
 
 template <int NP, bool FWD, bool MIXED>
-__global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
+__global__ __launch_bounds__(AShape<NP>::BLK)
+__attribute__((amdgpu_waves_per_eu(MIXED ? 1 : 3))) void k_adj_m2l_t(
     const int32_t *__restrict__ off, const uint64_t *__restrict__ keys, int sb,
     const double *__restrict__ ncoord, const void *__restrict__ mpv,
     const double *__restrict__ lbar, const double *__restrict__ pavg4, double *__restrict__ Xbar,
@@ -542,16 +556,16 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
     using T = typename std::conditional<MIXED, float, double>::type;
     using V4 = typename std::conditional<MIXED, float4, double4>::type;
     constexpr int NN = AShape<NP>::NN, BLK = AShape<NP>::BLK, TPT = AShape<NP>::TPT;
-    constexpr int SH = (4 * NN > 3 * BLK) ? 4 * NN : 3 * BLK;
+    // the final reductions (yb, pr3; BLK >= NN) and, with MIXED, the staged source
+    constexpr int SH = (MIXED && 2 * NN > 3 * BLK) ? 2 * NN : 3 * BLK;
     constexpr int NA = 17;  // accumulators: F[6], G, A[3], sum c R [3], sum c pr, sum c pr R [3]
     constexpr bool YS = YSplit<NP>::on;
     static_assert(!YS || TPT == 1, "source-node sums assume one target node per thread");
     constexpr int NY = YS ? 3 * NP : 1, KS = YS ? BLK / 16 : 1;
     __shared__ double shb[SH];
-    __shared__ double sn[3 * NP];
+    __shared__ double lsn[MIXED ? 3 * NP : 1];
     __shared__ double ybuf[NY][YS ? BLK : 1];  // per lane: column, row, plane sums
     __shared__ double ybr[NY][KS];             // sums over 16 lanes
-    V4 *smp = reinterpret_cast<V4 *>(shb);
     double(*yb)[3] = reinterpret_cast<double(*)[3]>(shb);
     double(*pr3)[3] = reinterpret_cast<double(*)[3]>(shb);
     const V4 *mp = reinterpret_cast<const V4 *>(mpv);
@@ -583,11 +597,20 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
     if (YS && tid >= NN)
         for (int j = 0; j < NY; ++j) ybuf[j][tid] = 0.0;
     for (int32_t ei = beg; ei < end; ++ei) {
+        // the source's multipoles and node coordinates: MIXED stages them in LDS;
+        // FP64 reads them with wave-uniform addresses (scalar loads)
         const int64_t s = (int64_t)(keys[ei] & mask);
-        __syncthreads();
-        for (int j = tid; j < NN; j += BLK) smp[j] = mp[s * NN + j];
-        if (tid < 3 * NP) sn[tid] = ncoord[s * 3 * NP + tid];
-        __syncthreads();
+        const V4 *smp = mp + s * NN;
+        const double *sn = ncoord + s * 3 * NP;
+        if constexpr (MIXED) {
+            V4 *lmp = reinterpret_cast<V4 *>(shb);
+            __syncthreads();
+            for (int j = tid; j < NN; j += BLK) lmp[j] = smp[j];
+            if (tid < 3 * NP) lsn[tid] = sn[tid];
+            __syncthreads();
+            smp = lmp;
+            sn = lsn;
+        }
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             if (tid + q * BLK >= NN) continue;
@@ -714,7 +737,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_t(
                 for (int si = 0; si < NP; ++si) ybuf[si][tid] = 3.0 * (double)Xs[si];
             }
         }
-        if constexpr (YS) {  // lanes -> sums over 16 lanes; finished at the next pair
+        if constexpr (YS) {  // lanes -> sums over 16 lanes -> the pair's sums
             __syncthreads();
             for (int w = tid; w < NY * KS; w += BLK) {
                 const int j = w / KS, k = w % KS;
@@ -801,7 +824,7 @@ template <int NP, bool MIXED>
 __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
     const int32_t *__restrict__ offT, const uint64_t *__restrict__ keysT, int sb,
     const double *__restrict__ ncoord, const void *__restrict__ mpv,
-    const double *__restrict__ lbar, const double *__restrict__ pavg4, double *__restrict__ mbar,
+    const void *__restrict__ lbarv, const double *__restrict__ pavg4, double *__restrict__ mbar,
     double *__restrict__ Xbar, const double *__restrict__ ysp) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using V4 = typename std::conditional<MIXED, float4, double4>::type;
@@ -809,11 +832,14 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
     constexpr bool YS = YSplit<NP>::on;
     // gb[4]; without YS also sum y3 e [3], sum y3 b [3], sum c R [3], sum c pr
     constexpr int NA = YS ? 4 : 14;
-    __shared__ double shl[NN * 6];
-    __shared__ double tn[3 * NP];
-    T *slb = reinterpret_cast<T *>(shl);
+    // MIXED: the target's cotangents staged in LDS as FP32; FP64: read with
+    // wave-uniform addresses (scalar loads)
+    constexpr int SL = (MIXED || !YS) ? 3 * NN : 1;  // staged FP32 cotangents / yb
+    __shared__ double shl[SL];
+    __shared__ double ltn[MIXED ? 3 * NP : 1];
     double(*yb)[3] = reinterpret_cast<double(*)[3]>(shl);
     const V4 *mp = reinterpret_cast<const V4 *>(mpv);
+    const double *lbar = reinterpret_cast<const double *>(lbarv);
     const int tid = threadIdx.x;
     const int32_t s = xcd_item(blockIdx.x, gridDim.x);
     const int32_t beg = offT[s], end = offT[s + 1];
@@ -833,11 +859,22 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
         for (int k = 0; k < NA; ++k) ac[q][k] = 0, ad[q][k] = 0.0;
     }
     for (int32_t ei = beg; ei < end; ++ei) {
+        // the target's node coordinates and local cotangents are the same for every
+        // lane: wave-uniform addresses (scalar loads, no LDS staging)
         const int64_t t = (int64_t)(keysT[ei] & mask);
-        __syncthreads();
-        for (int j = tid; j < NN * 6; j += BLK) slb[j] = (T)lbar[t * NN * 6 + j];
-        if (tid < 3 * NP) tn[tid] = ncoord[t * 3 * NP + tid];
-        __syncthreads();
+        const double *tn = ncoord + t * 3 * NP;
+        const T *tlb;
+        if constexpr (MIXED) {
+            float *lf = reinterpret_cast<float *>(shl);
+            __syncthreads();
+            for (int j = tid; j < NN * 6; j += BLK) lf[j] = (float)lbar[t * NN * 6 + j];
+            if (tid < 3 * NP) ltn[tid] = tn[tid];
+            __syncthreads();
+            tlb = lf;
+            tn = ltn;
+        } else {
+            tlb = lbar + t * NN * 6;
+        }
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             if (tid + q * BLK >= NN) continue;
@@ -865,7 +902,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
                         if constexpr (MIXED) y = __builtin_amdgcn_rsqf(d);
                         else y = rsqrt_nr(d);
                         const T y2 = y * y, y3 = y2 * y, uu = y3 * dx;
-                        const T *o = slb + 6 * (ti + NP * (tj + NP * tk));
+                        const T *o = tlb + 6 * (ti + NP * (tj + NP * tk));
                         const T e0 = o[0], e1 = o[1], e2 = o[2], b0 = o[3], b1 = o[4], b2 = o[5];
                         Sxe = fma(uu, e0, Sxe);
                         Ey = fma(y3, e1, Ey);
@@ -910,7 +947,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
     }
     if constexpr (YS) {
         const int node = tid;
-        if (node < NN) {
+        if (node < NN) {  // (no LDS is read below: no barrier needed)
             const double *S = MIXED ? ad[0] : nullptr;
             double *o = mbar + ((int64_t)s * NN + node) * 4;
             for (int d = 0; d < 4; ++d) o[d] = MIXED ? S[d] : (double)ac[0][d];

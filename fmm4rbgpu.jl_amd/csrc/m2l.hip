@@ -7,8 +7,10 @@
 //   K(R; p) = R / sqrt(R.R + (p.R)^2 + eps)^3.
 //
 // One workgroup owns one target cluster and walks its CSR source list; one
-// thread owns one target node.  Each source's multipoles are staged in LDS
-// and read as broadcasts.  The node grid is a tensor product, so for a fixed
+// thread owns one target node.  Each source's multipoles and node coordinates
+// are the same for all lanes: the mixed-precision kernel stages them in LDS and
+// reads broadcasts; the FP64 kernel reads them with wave-uniform addresses,
+// which become scalar loads (the scalar cache instead of LDS bandwidth).  The node grid is a tensor product, so for a fixed
 // source row (j, k) the y/z parts of R.R and p.R are hoisted and the i loop
 // needs three operations of geometry per evaluation; E and B are accumulated
 // through the row sums G = sum y3 g, H = sum y3 dx g, A = sum y3 m,
@@ -16,8 +18,8 @@
 // instead of 12.  In the mixed-precision form the row geometry is formed in
 // FP64 and rounded once to FP32, the kernel and row sums run in FP32
 // (v_rsq_f32), and each source pair's FP32 sum is added into FP64
-// accumulators.  The reference's LDS reuse without a barrier
-// (gpu_interaction.jl:15-38) is not repeated: a barrier precedes every refill.
+// accumulators.  (The reference stages the source in shared memory,
+// gpu_interaction.jl:15-38.)
 
 #include "fmm4rb_internal.hpp"
 
@@ -126,8 +128,11 @@ __global__ __launch_bounds__(M2LShape<NP>::BLK) void k_m2l(
     using T = typename std::conditional<MIXED, float, double>::type;
     using V4 = typename std::conditional<MIXED, float4, double4>::type;
     constexpr int NN = M2LShape<NP>::NN, BLK = M2LShape<NP>::BLK, TPT = M2LShape<NP>::TPT;
-    __shared__ V4 smp[NN];
-    __shared__ double sn[3 * NP];
+    // MIXED: the source staged in LDS (16 B per node read per pair); FP64: read
+    // with wave-uniform addresses through the scalar cache (32 B per node per pair
+    // would take most of the LDS bandwidth)
+    __shared__ V4 lmp[MIXED ? NN : 1];
+    __shared__ double lsn[MIXED ? 3 * NP : 1];
     const V4 *mp = reinterpret_cast<const V4 *>(mpv);
     const int tid = threadIdx.x;
     const int32_t t = xcd_item(blockIdx.x, gridDim.x);
@@ -147,10 +152,16 @@ __global__ __launch_bounds__(M2LShape<NP>::BLK) void k_m2l(
     }
     for (int32_t e = beg; e < end; ++e) {
         const int64_t s = (int64_t)(keys[e] & mask);
-        __syncthreads();
-        for (int j = tid; j < NN; j += BLK) smp[j] = mp[s * NN + j];
-        if (tid < 3 * NP) sn[tid] = ncoord[s * 3 * NP + tid];
-        __syncthreads();
+        const double *sn = ncoord + s * 3 * NP;
+        const V4 *smp = mp + s * NN;
+        if constexpr (MIXED) {
+            __syncthreads();
+            for (int j = tid; j < NN; j += BLK) lmp[j] = smp[j];
+            if (tid < 3 * NP) lsn[tid] = sn[tid];
+            __syncthreads();
+            sn = lsn;
+            smp = lmp;
+        }
 #pragma unroll
         for (int q = 0; q < TPT; ++q)
             if (tid + q * BLK < NN) m2l_pair<NP, T, V4>(tx[q], ty[q], tz[q], sn, smp, px, py, pz, acc[q]);
