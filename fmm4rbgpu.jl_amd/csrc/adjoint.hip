@@ -545,9 +545,17 @@ struct YSplit {
     static constexpr bool on = NP <= 6;  // 3 NP x BLK doubles of LDS <= 36 KB
 };
 
+// Occupancy: the FP64 form with the source-node sums (4 <= NP <= 6) is held to 3 waves
+// per SIMD (168 VGPRs, no spills): 167.5 -> 162.6 ms at N = 2^22, n = 4.  At NP = 7 the
+// same bound made it 15% slower (rematerialisation), so larger NP keep the default.
+template <int NP, bool MIXED>
+struct AdjTWaves {
+    static constexpr int v = (!MIXED && NP >= 4 && YSplit<NP>::on) ? 3 : 1;
+};
+
 template <int NP, bool FWD, bool MIXED>
 __global__ __launch_bounds__(AShape<NP>::BLK)
-__attribute__((amdgpu_waves_per_eu(MIXED ? 1 : 3))) void k_adj_m2l_t(
+__attribute__((amdgpu_waves_per_eu(AdjTWaves<NP, MIXED>::v))) void k_adj_m2l_t(
     const int32_t *__restrict__ off, const uint64_t *__restrict__ keys, int sb,
     const double *__restrict__ ncoord, const void *__restrict__ mpv,
     const double *__restrict__ lbar, const double *__restrict__ pavg4, double *__restrict__ Xbar,
