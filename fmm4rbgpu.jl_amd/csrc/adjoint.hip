@@ -1339,15 +1339,16 @@ __global__ void k_fill0(double *__restrict__ p, int64_t n) {
 
 // dl/d(bbox) per node: a_d (min) and b_d (max) through X_m = (c_m (b-a))/2 + (a+b)/2.
 __global__ void k_adj_bbox(int64_t nc, int np, const uint8_t *__restrict__ flags,
-                           const double *__restrict__ Xbar, const double *__restrict__ ctab,
-                           double *__restrict__ abar) {
+                           const double *__restrict__ Xbar, const double *__restrict__ Xbar2,
+                           const double *__restrict__ ctab, double *__restrict__ abar) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nc) return;
     for (int d = 0; d < 3; ++d) {
         double sa = 0.0, sbm = 0.0;
         if (flags[k] & 1)
             for (int m = 0; m < np; ++m) {
-                const double xb = Xbar[(k * 3 + d) * np + m];
+                const int64_t j = (k * 3 + d) * np + m;
+                const double xb = Xbar[j] + Xbar2[j];
                 sa += xb * ((1.0 - ctab[m]) / 2.0);
                 sbm += xb * ((1.0 + ctab[m]) / 2.0);
             }
@@ -1433,6 +1434,7 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     c.d_lbar.ensure(6 * (size_t)NN * nc * 8);
     c.d_mbar.ensure(4 * (size_t)NN * nc * 8);
     c.d_xbar.ensure(3 * (size_t)np * nc * 8);
+    c.d_xbar2.ensure(3 * (size_t)np * nc * 8);
     c.d_pbar.ensure((3 * (size_t)nc + 3) * 8);
     c.d_abar.ensure(6 * (size_t)nc * 8);
     c.d_costab.ensure(MAX_NP * 8);
@@ -1493,6 +1495,9 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     const int64_t nx = 3 * np * nc;
     hipLaunchKernelGGL(k_fill0, dim3(nbk(nx, 256)), dim3(256), 0, st, xbar, nx);
     FMM_LAUNCH_CHECK();
+    double *xbar2 = c.d_xbar2.as<double>();
+    hipLaunchKernelGGL(k_fill0, dim3(nbk(nx, 256)), dim3(256), 0, st, xbar2, nx);
+    FMM_LAUNCH_CHECK();
     double *loc = c.d_loc.as<double>(), *pbar = c.d_pbar.as<double>();
     // mixed precision: the M2L^T kernels read the float4 copy of the multipoles
     if (mixed) launch_mp_to_f32(c);
@@ -1531,10 +1536,15 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     FMM_LAUNCH_CHECK();
     launch_l2l(c);
     FMM_HIP(hipEventRecord(c.ev[8], st));
-    // 3. geometry of L2P^T and L2L^T (needs the final forward locals); the L2L^T
-    //    geometry of all levels in two launches (parents on even, then odd levels)
+    // 3. geometry of L2P^T and L2L^T (needs the final forward locals; latency-bound
+    //    launches): on the third stream, beside the M2L^T source side (4.), which
+    //    adds its node-coordinate terms into xbar2; the L2L^T geometry of all
+    //    levels in two launches (parents on even, then odd levels)
+    hipStream_t sg = c.aux2;
+    FMM_HIP(hipEventRecord(c.evy[0], st));
+    FMM_HIP(hipStreamWaitEvent(sg, c.evy[0], 0));
 #define ADJ_L2PG(NP)                                                                                \
-    hipLaunchKernelGGL((k_adj_leaf<NP, 0, 1>), dim3(nleaf), dim3(AT), 0, st, leaves, lo, cnt, sp8,  \
+    hipLaunchKernelGGL((k_adj_leaf<NP, 0, 1>), dim3(nleaf), dim3(AT), 0, sg, leaves, lo, cnt, sp8,  \
                        ncoord, loc, ob, lbar, gx, gp, xbar)
     ADJ_NP_SWITCH(c.np, ADJ_L2PG);
     FMM_LAUNCH_CHECK();
@@ -1542,21 +1552,22 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     const int32_t *allpar = c.d_lvl_split.as<int32_t>();
     for (int parity = 0; parity < 2 && npar > 0; ++parity) {
 #define ADJ_L2LG(NP)                                                                               \
-    hipLaunchKernelGGL((k_adj_transfer<NP, 0, 1>), dim3((unsigned)npar), dim3(AT), 0, st, allpar, \
+    hipLaunchKernelGGL((k_adj_transfer<NP, 0, 1>), dim3((unsigned)npar), dim3(AT), 0, sg, allpar, \
                        ncoord, lbar, loc, lbar, xbar, parity)
         ADJ_NP_SWITCH(c.np, ADJ_L2LG);
         FMM_LAUNCH_CHECK();
     }
+    FMM_HIP(hipEventRecord(c.evy[1], sg));
     // 4. M2L^T source side
 #define ADJ_M2LS(NP)                                                                              \
     if (mixed)                                                                                    \
         hipLaunchKernelGGL((k_adj_m2l_s<NP, true>), dim3((unsigned)nc), dim3(AShape<NP>::BLK), 0, \
                            st, c.d_m2lT_off.as<int32_t>(), c.m2lT_sorted, c.sb, ncoord, mpv, lbar, \
-                           c.d_pavg.as<double>(), mbar, xbar, ysp);                               \
+                           c.d_pavg.as<double>(), mbar, xbar2, ysp);                              \
     else                                                                                          \
         hipLaunchKernelGGL((k_adj_m2l_s<NP, false>), dim3((unsigned)nc), dim3(AShape<NP>::BLK),   \
                            0, st, c.d_m2lT_off.as<int32_t>(), c.m2lT_sorted, c.sb, ncoord, mpv,    \
-                           lbar, c.d_pavg.as<double>(), mbar, xbar, ysp)
+                           lbar, c.d_pavg.as<double>(), mbar, xbar2, ysp)
     ADJ_NP_SWITCH(c.np, ADJ_M2LS);
     FMM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_adj_pbar, dim3(1), dim3(256), 0, st, pbar, nc, pbar + 3 * nc);
@@ -1574,6 +1585,7 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
         ADJ_NP_SWITCH(c.np, ADJ_M2M);
         FMM_LAUNCH_CHECK();
     }
+    FMM_HIP(hipStreamWaitEvent(st, c.evy[1], 0));  // join: the geometry below also adds to xbar
     for (int parity = 0; parity < 2 && npar > 0; ++parity) {
 #define ADJ_M2MG(NP)                                                                               \
     hipLaunchKernelGGL((k_adj_transfer<NP, 1, 1>), dim3((unsigned)npar), dim3(AT), 0, st, allpar, \
@@ -1589,7 +1601,7 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     FMM_LAUNCH_CHECK();
     // bounding boxes -> extreme particles, mean momentum, output order
     hipLaunchKernelGGL(k_adj_bbox, dim3(nbk(nc, 128)), dim3(128), 0, st, nc, c.np,
-                       c.d_flags.as<uint8_t>(), xbar, c.d_costab.as<double>(),
+                       c.d_flags.as<uint8_t>(), xbar, xbar2, c.d_costab.as<double>(),
                        c.d_abar.as<double>());
     FMM_LAUNCH_CHECK();
     FMM_HIP(hipStreamWaitEvent(st, c.evx[2], 0));  // join the P2P^T stream

@@ -410,9 +410,11 @@ int fmm4rb_create(fmm4rb_ctx **out, int device) {
     try {
         FMM_HIP(hipStreamCreateWithFlags(&ctx->c.own, hipStreamNonBlocking));
         FMM_HIP(hipStreamCreateWithFlags(&ctx->c.aux, hipStreamNonBlocking));
+        FMM_HIP(hipStreamCreateWithFlags(&ctx->c.aux2, hipStreamNonBlocking));
         ctx->c.stream = ctx->c.own;
         for (auto &e : ctx->c.ev) FMM_HIP(hipEventCreate(&e));
         for (auto &e : ctx->c.evx) FMM_HIP(hipEventCreate(&e));
+        for (auto &e : ctx->c.evy) FMM_HIP(hipEventCreate(&e));
     } catch (...) {
         delete ctx;
         throw;
@@ -427,13 +429,14 @@ int fmm4rb_destroy(fmm4rb_ctx *ctx) {
     (void)hipSetDevice(ctx->c.device);
     if (ctx->c.own) (void)hipStreamSynchronize(ctx->c.own);
     if (ctx->c.aux) (void)hipStreamSynchronize(ctx->c.aux);
+    if (ctx->c.aux2) (void)hipStreamSynchronize(ctx->c.aux2);
     Ctx &c = ctx->c;
     DevBuf *bufs[] = {&c.d_lo, &c.d_cnt, &c.d_flags, &c.d_leaves, &c.d_lvl_nodes, &c.d_lvl_split,
                       &c.d_leftbase, &c.d_part, &c.d_pavg, &c.d_bad, &c.d_keys, &c.d_keys_alt,
                       &c.d_vals, &c.d_vals_alt, &c.d_rs_scratch, &c.d_seg, &c.d_seg_alt,
                       &c.d_rk, &c.d_pivot, &c.d_bsum, &c.d_axis, &c.d_parind, &c.d_leafof,
                       &c.d_bbox, &c.d_geo, &c.d_ncoord, &c.d_costab, &c.d_ext, &c.d_sp8, &c.d_sp8f,
-                      &c.d_ob, &c.d_gx, &c.d_gp, &c.d_gx2, &c.d_gp2, &c.d_lbar, &c.d_mbar,
+                      &c.d_ob, &c.d_gx, &c.d_gp, &c.d_gx2, &c.d_gp2, &c.d_xbar2, &c.d_lbar, &c.d_mbar,
                       &c.d_xbar, &c.d_pbar,
                       &c.d_abar, &c.d_p2pT, &c.d_p2pT_alt, &c.d_m2lT, &c.d_m2lT_alt,
                       &c.d_p2pT_off, &c.d_m2lT_off,
@@ -446,8 +449,11 @@ int fmm4rb_destroy(fmm4rb_ctx *ctx) {
         if (e) (void)hipEventDestroy(e);
     for (auto &e : c.evx)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : c.evy)
+        if (e) (void)hipEventDestroy(e);
     if (c.own) (void)hipStreamDestroy(c.own);
     if (c.aux) (void)hipStreamDestroy(c.aux);
+    if (c.aux2) (void)hipStreamDestroy(c.aux2);
     delete ctx;
     FMM_API_END
 }

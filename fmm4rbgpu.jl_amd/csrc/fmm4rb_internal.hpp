@@ -150,6 +150,10 @@ struct Ctx {
     // evx[0] fork, evx[1] P2P start, evx[2] P2P end (the join)
     hipStream_t aux = nullptr;
     hipEvent_t ev[10] = {}, evx[3] = {};
+    // reverse mode: the L2P^T/L2L^T geometry runs on aux2 beside the M2L^T source
+    // side (evy[0] fork, evy[1] join)
+    hipStream_t aux2 = nullptr;
+    hipEvent_t evy[2] = {};
     bool debug = false;
 
     // parameters of the current call
@@ -179,6 +183,7 @@ struct Ctx {
     // adjoint (field_vjp)
     DevBuf d_ob, d_gx, d_gp, d_lbar, d_mbar, d_xbar, d_pbar, d_abar;
     DevBuf d_gx2, d_gp2;  // P2P^T gradients (written on aux)
+    DevBuf d_xbar2;       // node-coordinate gradients of the M2L^T source side
     DevBuf d_p2pT, d_p2pT_alt, d_m2lT, d_m2lT_alt, d_p2pT_off, d_m2lT_off;
     DevBuf d_m2lT_pos, d_ysp;  // transposed position of each M2L pair; per-pair source sums
     uint64_t *p2pT_sorted = nullptr, *m2lT_sorted = nullptr;

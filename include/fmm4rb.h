@@ -55,8 +55,9 @@ const char *fmm4rb_version(void);
 const char *fmm4rb_last_error(void);
 
 /* No reference counterpart: the reference uses CUDA.jl's implicit context. */
-/* A context owns two HIP streams (default priority): its own, and one for the
- * near field, forked from and joined back into the stream the call runs on. */
+/* A context owns three HIP streams (default priority): its own, one for the near
+ * field, and one for the reverse mode's L2P^T/L2L^T geometry, each forked from and
+ * joined back into the stream the call runs on. */
 int fmm4rb_create(fmm4rb_ctx **ctx, int device);
 int fmm4rb_destroy(fmm4rb_ctx *ctx);
 /* Run the context's work on a caller stream (hipStream_t; NULL restores the
@@ -94,7 +95,8 @@ int fmm4rb_field_device(fmm4rb_ctx *ctx, int64_t N, const double *d_pos, const d
  * transposed lists, local cotangents and the
  * forward M2L fused with the M2L^T target side plus the forward L2L; ms_p2p =
  * P2P^T (both sides, on the near-field stream); ms_downward = L2P^T/L2L^T
- * geometry, the M2L^T source side, M2M^T, P2M^T and the gradient assembly.
+ * geometry (on the third stream) beside the M2L^T source side, then M2M^T, P2M^T
+ * and the gradient assembly.
  */
 int fmm4rb_field_vjp(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
                      double charge, double lambda, int n, int64_t N0, double eta, int precision,
