@@ -1499,8 +1499,8 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     hipLaunchKernelGGL(k_fill0, dim3(nbk(nx, 256)), dim3(256), 0, st, xbar2, nx);
     FMM_LAUNCH_CHECK();
     double *loc = c.d_loc.as<double>(), *pbar = c.d_pbar.as<double>();
-    // mixed precision: the M2L^T kernels read the float4 copy of the multipoles
-    if (mixed) launch_mp_to_f32(c);
+    // mixed precision: the M2L^T kernels read the float4 copy of the multipoles,
+    // made by the upward pass
     const void *mpv = mixed ? (const void *)c.d_mp4f.p : (const void *)c.d_mp4.p;
 
     // 1. local cotangents (values only; they do not depend on the forward locals):

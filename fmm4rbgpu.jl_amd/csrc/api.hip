@@ -231,6 +231,18 @@ static void stage_pavg(Ctx &c, const double *pos, const double *mom) {
     if (bad) fail(FMM4RB_EINVAL, "positions or momenta contain NaN or Inf");
 }
 
+// The upward pass (P2M, M2M; for mixed precision also the FP32 copy of the
+// multipoles) needs only the tree: it runs on the third stream, beside the list
+// build on the call's stream, which joins it (evu[2]) before the far field.
+static void fork_upward(Ctx &c) {
+    FMM_HIP(hipEventRecord(c.evu[0], c.stream));
+    FMM_HIP(hipStreamWaitEvent(c.aux2, c.evu[0], 0));
+    FMM_HIP(hipEventRecord(c.evu[1], c.aux2));
+    launch_upward(c, c.aux2);
+    if (c.precision == FMM4RB_MIXED) launch_mp_to_f32(c, c.aux2);
+    FMM_HIP(hipEventRecord(c.evu[2], c.aux2));
+}
+
 static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, double charge,
                       double lambda, int n, int64_t N0, double eta, int precision, double *E,
                       double *B, fmm4rb_stats *stats) {
@@ -246,6 +258,7 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
     build_tree_device(c, pos, mom, N, N0);
     launch_node_geo(c);
     FMM_HIP(hipEventRecord(c.ev[2], st));
+    fork_upward(c);
     build_lists_device(c, eta);
     FMM_HIP(hipEventRecord(c.ev[3], st));
     // the near field needs only the tree order and the lists: it runs on the
@@ -255,8 +268,7 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
     FMM_HIP(hipEventRecord(c.evx[1], c.aux));
     launch_p2p(c, c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, c.aux);
     FMM_HIP(hipEventRecord(c.evx[2], c.aux));
-    launch_upward(c);
-    if (precision == FMM4RB_MIXED) launch_mp_to_f32(c);
+    FMM_HIP(hipStreamWaitEvent(st, c.evu[2], 0));  // join the upward pass
     FMM_HIP(hipEventRecord(c.ev[4], st));
     launch_m2l(c, c.d_m2l_off.as<int32_t>(), c.m2l_sorted, c.sb, c.shape.nc);
     FMM_HIP(hipEventRecord(c.ev[5], st));
@@ -307,7 +319,7 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
         FMM_HIP(hipEventElapsedTime(&stats->ms_pavg, c.ev[0], c.ev[1]));
         FMM_HIP(hipEventElapsedTime(&stats->ms_tree, c.ev[1], c.ev[2]));
         FMM_HIP(hipEventElapsedTime(&stats->ms_lists, c.ev[2], c.ev[3]));
-        FMM_HIP(hipEventElapsedTime(&stats->ms_upward, c.ev[3], c.ev[4]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_upward, c.evu[1], c.evu[2]));
         FMM_HIP(hipEventElapsedTime(&stats->ms_m2l, c.ev[4], c.ev[5]));
         FMM_HIP(hipEventElapsedTime(&stats->ms_p2p, c.evx[1], c.evx[2]));
         FMM_HIP(hipEventElapsedTime(&stats->ms_downward, c.ev[5], c.ev[7]));
@@ -331,9 +343,10 @@ static void run_vjp(Ctx &c, int64_t N, const double *pos, const double *mom, dou
     build_tree_device(c, pos, mom, N, N0);
     launch_node_geo(c);
     FMM_HIP(hipEventRecord(c.ev[2], st));
-    launch_upward(c);
-    FMM_HIP(hipEventRecord(c.ev[3], st));
+    fork_upward(c);
     build_lists_device(c, eta);
+    FMM_HIP(hipEventRecord(c.ev[3], st));
+    FMM_HIP(hipStreamWaitEvent(st, c.evu[2], 0));  // join the upward pass
     FMM_HIP(hipEventRecord(c.ev[4], st));
     FMM_HIP(hipEventRecord(c.ev[5], st));
     run_adjoint(c, Ebar, Bbar, amp, gpos, gmom);
@@ -356,8 +369,8 @@ static void run_vjp(Ctx &c, int64_t N, const double *pos, const double *mom, dou
         FMM_HIP(hipEventElapsedTime(&stats->ms_total, c.ev[0], c.ev[6]));
         FMM_HIP(hipEventElapsedTime(&stats->ms_pavg, c.ev[0], c.ev[1]));
         FMM_HIP(hipEventElapsedTime(&stats->ms_tree, c.ev[1], c.ev[2]));
-        FMM_HIP(hipEventElapsedTime(&stats->ms_upward, c.ev[2], c.ev[3]));
-        FMM_HIP(hipEventElapsedTime(&stats->ms_lists, c.ev[3], c.ev[4]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_upward, c.evu[1], c.evu[2]));
+        FMM_HIP(hipEventElapsedTime(&stats->ms_lists, c.ev[2], c.ev[3]));
         FMM_HIP(hipEventElapsedTime(&stats->ms_m2l, c.ev[5], c.ev[8]));
         FMM_HIP(hipEventElapsedTime(&stats->ms_p2p, c.evx[1], c.evx[2]));
         FMM_HIP(hipEventElapsedTime(&stats->ms_downward, c.ev[8], c.ev[6]));
@@ -415,6 +428,7 @@ int fmm4rb_create(fmm4rb_ctx **out, int device) {
         for (auto &e : ctx->c.ev) FMM_HIP(hipEventCreate(&e));
         for (auto &e : ctx->c.evx) FMM_HIP(hipEventCreate(&e));
         for (auto &e : ctx->c.evy) FMM_HIP(hipEventCreate(&e));
+        for (auto &e : ctx->c.evu) FMM_HIP(hipEventCreate(&e));
     } catch (...) {
         delete ctx;
         throw;
@@ -450,6 +464,8 @@ int fmm4rb_destroy(fmm4rb_ctx *ctx) {
     for (auto &e : c.evx)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : c.evy)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : c.evu)
         if (e) (void)hipEventDestroy(e);
     if (c.own) (void)hipStreamDestroy(c.own);
     if (c.aux) (void)hipStreamDestroy(c.aux);
@@ -775,7 +791,7 @@ int fmm4rb_op_m2l(fmm4rb_ctx *ctx, int n, int precision, int64_t nc, const doubl
     FMM_HIP(hipMemcpyAsync(c.d_bbox.p, bboxes, 48 * (size_t)nc, hipMemcpyHostToDevice, st));
     FMM_HIP(hipMemcpyAsync(c.d_mp4.p, mp.data(), 8 * mp.size(), hipMemcpyHostToDevice, st));
     launch_node_geo(c);
-    if (precision == FMM4RB_MIXED) launch_mp_to_f32(c);
+    if (precision == FMM4RB_MIXED) launch_mp_to_f32(c, c.stream);
     uint64_t *sorted = nullptr;
     hook_pairs(c, nc, npairs, pairs, c.d_m2l, c.d_m2l_alt, c.d_m2l_off, &sorted);
     c.cap_m2l = std::max<int64_t>(c.cap_m2l, (int64_t)(c.d_m2l.cap / 8));

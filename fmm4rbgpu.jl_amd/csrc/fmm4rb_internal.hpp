@@ -121,11 +121,11 @@ void sort_pairs_csr(Ctx &c, uint64_t *keys, uint64_t *keys_alt, int64_t n, int32
 
 // ---- interpolation operators (interp.hip)
 void launch_node_geo(Ctx &c);
-void launch_upward(Ctx &c);
+void launch_upward(Ctx &c, hipStream_t st);
 void launch_downward(Ctx &c, double amp, double *E, double *B);
 void launch_l2l(Ctx &c);
 void launch_l2p(Ctx &c, double amp, double *E, double *B);
-void launch_mp_to_f32(Ctx &c);
+void launch_mp_to_f32(Ctx &c, hipStream_t st);
 
 // ---- interactions (m2l.hip, p2p.hip)
 void launch_m2l(Ctx &c, const int32_t *off, const uint64_t *keys, int sb, int64_t ngroups_grid);
@@ -154,6 +154,8 @@ struct Ctx {
     // side (evy[0] fork, evy[1] join)
     hipStream_t aux2 = nullptr;
     hipEvent_t evy[2] = {};
+    // the upward pass runs on aux2 beside the list build: evu[0] fork, [1] start, [2] end
+    hipEvent_t evu[3] = {};
     bool debug = false;
 
     // parameters of the current call

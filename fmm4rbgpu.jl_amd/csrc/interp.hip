@@ -369,9 +369,8 @@ void launch_node_geo(Ctx &c) {
     FMM_LAUNCH_CHECK();
 }
 
-void launch_upward(Ctx &c) {
+void launch_upward(Ctx &c, hipStream_t st) {
     const Shape &S = c.shape;
-    hipStream_t st = c.stream;
     const unsigned nleaf = (unsigned)S.leaves.size();
 #define P2M_L(NP)                                                                            \
     hipLaunchKernelGGL(k_p2m<NP>, dim3(nleaf), dim3(OP_T), 0, st, c.d_leaves.as<int32_t>(), \
@@ -392,9 +391,9 @@ void launch_upward(Ctx &c) {
     }
 }
 
-void launch_mp_to_f32(Ctx &c) {
+void launch_mp_to_f32(Ctx &c, hipStream_t st) {
     const int64_t n = c.shape.nc * (int64_t)c.nn;
-    hipLaunchKernelGGL(k_mp_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, n,
+    hipLaunchKernelGGL(k_mp_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
                        c.d_mp4.as<double>(), c.d_mp4f.as<float>());
     FMM_LAUNCH_CHECK();
 }
