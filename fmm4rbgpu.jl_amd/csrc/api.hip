@@ -125,6 +125,10 @@ static void validate(int64_t N, int n, int64_t N0, double eta, int precision) {
 // Set parameters, (re)compute the shape and size every buffer of the pipeline.
 static void prepare(Ctx &c, int64_t N, int64_t N0, int n, int precision) {
     hipStream_t st = c.stream;
+    // a call that failed after forking work to the helper streams may have left it
+    // running: drain them before any buffer is reused (no cost when idle)
+    FMM_HIP(hipStreamSynchronize(c.aux));
+    FMM_HIP(hipStreamSynchronize(c.aux2));
     if (c.shape.N != N || c.shape.N0 != N0) {
         compute_shape(c.shape, N, N0);
         FMM_HIP(hipStreamSynchronize(st));  // host vectors may be in flight
