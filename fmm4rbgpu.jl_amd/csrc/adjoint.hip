@@ -1465,7 +1465,7 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     // P2P^T needs only the particles, the cotangents and the near-field lists: it
     // runs on the auxiliary stream into gx2/gp2 while this stream does the far field
     {
-        hipStream_t sa = c.aux;
+        hipStream_t sa = c.near_stream();
         FMM_HIP(hipEventRecord(c.evx[0], st));
         FMM_HIP(hipStreamWaitEvent(sa, c.evx[0], 0));
         FMM_HIP(hipEventRecord(c.evx[1], sa));
@@ -1540,7 +1540,7 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     //    launches): on the third stream, beside the M2L^T source side (4.), which
     //    adds its node-coordinate terms into xbar2; the L2L^T geometry of all
     //    levels in two launches (parents on even, then odd levels)
-    hipStream_t sg = c.aux2;
+    hipStream_t sg = c.third_stream();
     FMM_HIP(hipEventRecord(c.evy[0], st));
     FMM_HIP(hipStreamWaitEvent(sg, c.evy[0], 0));
 #define ADJ_L2PG(NP)                                                                                \

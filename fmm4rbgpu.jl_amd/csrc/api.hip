@@ -239,12 +239,13 @@ static void stage_pavg(Ctx &c, const double *pos, const double *mom) {
 // multipoles) needs only the tree: it runs on the third stream, beside the list
 // build on the call's stream, which joins it (evu[2]) before the far field.
 static void fork_upward(Ctx &c) {
+    hipStream_t s3 = c.third_stream();
     FMM_HIP(hipEventRecord(c.evu[0], c.stream));
-    FMM_HIP(hipStreamWaitEvent(c.aux2, c.evu[0], 0));
-    FMM_HIP(hipEventRecord(c.evu[1], c.aux2));
-    launch_upward(c, c.aux2);
-    if (c.precision == FMM4RB_MIXED) launch_mp_to_f32(c, c.aux2);
-    FMM_HIP(hipEventRecord(c.evu[2], c.aux2));
+    FMM_HIP(hipStreamWaitEvent(s3, c.evu[0], 0));
+    FMM_HIP(hipEventRecord(c.evu[1], s3));
+    launch_upward(c, s3);
+    if (c.precision == FMM4RB_MIXED) launch_mp_to_f32(c, s3);
+    FMM_HIP(hipEventRecord(c.evu[2], s3));
 }
 
 static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, double charge,
@@ -268,10 +269,11 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
     // the near field needs only the tree order and the lists: it runs on the
     // auxiliary stream while this one forms the far field, and L2P waits for it
     FMM_HIP(hipEventRecord(c.evx[0], st));
-    FMM_HIP(hipStreamWaitEvent(c.aux, c.evx[0], 0));
-    FMM_HIP(hipEventRecord(c.evx[1], c.aux));
-    launch_p2p(c, c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, c.aux);
-    FMM_HIP(hipEventRecord(c.evx[2], c.aux));
+    hipStream_t sn = c.near_stream();
+    FMM_HIP(hipStreamWaitEvent(sn, c.evx[0], 0));
+    FMM_HIP(hipEventRecord(c.evx[1], sn));
+    launch_p2p(c, c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, sn);
+    FMM_HIP(hipEventRecord(c.evx[2], sn));
     FMM_HIP(hipStreamWaitEvent(st, c.evu[2], 0));  // join the upward pass
     FMM_HIP(hipEventRecord(c.ev[4], st));
     launch_m2l(c, c.d_m2l_off.as<int32_t>(), c.m2l_sorted, c.sb, c.shape.nc);
@@ -482,6 +484,13 @@ int fmm4rb_set_stream(fmm4rb_ctx *ctx, void *stream) {
     FMM_API_BEGIN
     FMM_NEED(ctx, "ctx is NULL");
     ctx->c.stream = stream ? (hipStream_t)stream : ctx->c.own;
+    FMM_API_END
+}
+
+int fmm4rb_set_overlap(fmm4rb_ctx *ctx, int on) {
+    FMM_API_BEGIN
+    FMM_NEED(ctx, "ctx is NULL");
+    ctx->c.overlap = on != 0;
     FMM_API_END
 }
 

@@ -156,6 +156,11 @@ struct Ctx {
     hipEvent_t evy[2] = {};
     // the upward pass runs on aux2 beside the list build: evu[0] fork, [1] start, [2] end
     hipEvent_t evu[3] = {};
+    // overlap = false runs the helper streams' work in place on the call's stream
+    // (same order of results, bit-identical): every kernel then runs alone on the GPU
+    bool overlap = true;
+    hipStream_t near_stream() const { return overlap ? aux : stream; }
+    hipStream_t third_stream() const { return overlap ? aux2 : stream; }
     bool debug = false;
 
     // parameters of the current call

@@ -15,7 +15,9 @@ there is no CPU fallback: if the library is missing, loading fails loudly.
 from __future__ import annotations
 
 import ctypes as C
+import importlib.util
 import os
+import sys
 from dataclasses import dataclass
 
 import numpy as np
@@ -29,9 +31,10 @@ AMP_RE = 2.8179403699772166e-15  # update_particles_field.jl:28
 
 EXPORTS = (
     "fmm4rb_version", "fmm4rb_last_error", "fmm4rb_create", "fmm4rb_destroy",
-    "fmm4rb_set_stream", "fmm4rb_field", "fmm4rb_field_device", "fmm4rb_bruteforce",
-    "fmm4rb_tree", "fmm4rb_lists", "fmm4rb_op_m2l", "fmm4rb_op_p2p", "fmm4rb_set_debug",
-    "fmm4rb_debug_fetch", "fmm4rb_field_vjp", "fmm4rb_field_vjp_device", "fmm4rb_bruteforce_vjp",
+    "fmm4rb_set_stream", "fmm4rb_set_overlap", "fmm4rb_field", "fmm4rb_field_device",
+    "fmm4rb_bruteforce", "fmm4rb_tree", "fmm4rb_lists", "fmm4rb_op_m2l", "fmm4rb_op_p2p",
+    "fmm4rb_set_debug", "fmm4rb_debug_fetch", "fmm4rb_field_vjp", "fmm4rb_field_vjp_device",
+    "fmm4rb_bruteforce_vjp",
 )
 
 
@@ -76,6 +79,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise FileNotFoundError(
                 f"{LIB_PATH} is missing: build it with fmm4rbgpu.jl_amd/build.py")
+        # PyTorch-ROCm ships its own libamdhip64 (same soname, loaded by RPATH).  If
+        # this library is loaded first, /opt/rocm's HIP runtime comes in with it and a
+        # later ``import torch`` brings a second runtime that finds no GPU; if torch is
+        # loaded first, this library binds to torch's runtime and the two share
+        # devices, memory and streams.  So load torch first whenever it is installed.
+        if "torch" not in sys.modules and importlib.util.find_spec("torch") is not None:
+            import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         L.fmm4rb_version.restype = C.c_char_p
         L.fmm4rb_last_error.restype = C.c_char_p
@@ -83,6 +93,7 @@ def lib():
         L.fmm4rb_destroy.argtypes = [_vp]
         L.fmm4rb_set_stream.argtypes = [_vp, _vp]
         L.fmm4rb_set_debug.argtypes = [_vp, C.c_int]
+        L.fmm4rb_set_overlap.argtypes = [_vp, C.c_int]
         L.fmm4rb_field.argtypes = [_vp, _i64, _dp, _dp, C.c_double, C.c_double, C.c_int, _i64,
                                    C.c_double, C.c_int, _dp, _dp, C.POINTER(Stats)]
         L.fmm4rb_field_device.argtypes = [_vp, _i64, _vp, _vp, C.c_double, C.c_double, C.c_int,
@@ -159,6 +170,11 @@ class Context:
 
     def set_stream(self, stream_ptr: int | None):
         _check(lib().fmm4rb_set_stream(self._h, stream_ptr or None))
+
+    def set_overlap(self, on: bool):
+        """False: run the near field and the upward pass in place on the call's
+        stream (kernels one at a time; bit-identical results) for per-kernel timing."""
+        _check(lib().fmm4rb_set_overlap(self._h, int(on)))
 
     def set_debug(self, on: bool):
         _check(lib().fmm4rb_set_debug(self._h, int(on)))

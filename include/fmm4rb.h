@@ -63,6 +63,11 @@ int fmm4rb_destroy(fmm4rb_ctx *ctx);
 /* Run the context's work on a caller stream (hipStream_t; NULL restores the
  * context's own stream). */
 int fmm4rb_set_stream(fmm4rb_ctx *ctx, void *stream);
+/* No reference counterpart: on = 0 runs the helper streams' work in place on the
+ * call's stream, one kernel at a time (results bit-identical to the default on = 1,
+ * slower).  For timing and profiling single kernels, whose durations are otherwise
+ * stretched by the kernels that share the GPU with them. */
+int fmm4rb_set_overlap(fmm4rb_ctx *ctx, int on);
 
 /*
  * Field evaluation, replaces update_particles_field!(beam, FMMGPU(n,N0,eta); lambda)
