@@ -158,6 +158,21 @@ def main():
     ms_m, st_m, iso_m = timed(ctx, torch, a.N, bufs_m, a.n, a.N0, a.eta, "mixed", a.steps,
                               a.warmup, lam)
     res = {"library_device_bytes": {"forward_mixed": lib_bytes()}}
+    # tracking steps: the same evaluation on the tree and lists of the last call
+    # (fmm4rb_field_device_refit; bounding boxes, p_avg and expansions recomputed)
+    dpm, dmm, dEm, dBm = bufs_m
+    refit = lambda: ctx.field_device_refit(a.N, dpm.data_ptr(), dmm.data_ptr(), dEm.data_ptr(),
+                                           dBm.data_ptr(), -1.0, lam, a.n, a.N0, a.eta, "mixed")
+    ctx.field_device(a.N, dpm.data_ptr(), dmm.data_ptr(), dEm.data_ptr(), dBm.data_ptr(), -1.0,
+                     lam, a.n, a.N0, a.eta, "mixed")
+    refit()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st_r = [refit() for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    ms_r = (time.perf_counter() - t0) / a.steps
+    res["tracking_refit_mixed"] = {"ms_per_step": ms_r * 1e3, "particles_per_s": a.N / ms_r,
+                                   "stages_ms": stage_avg(st_r)}
     if not a.skip_fp64:
         ms_d, st_d, iso_d = timed(ctx, torch, a.N, bufs_d, a.n, a.N0, a.eta, "fp64", a.steps,
                                   a.warmup, lam)
@@ -226,7 +241,8 @@ def main():
         "stages_ms_isolated": stage_avg(iso_m),
         "roofline": roofline(s, stage_avg(iso_m), a.n, "mixed", a.N),
     }
-    for k in ("adjoint_fp64", "adjoint_mixed", "host_pointers_mixed", "library_device_bytes"):
+    for k in ("adjoint_fp64", "adjoint_mixed", "tracking_refit_mixed", "host_pointers_mixed",
+              "library_device_bytes"):
         if k in res:
             out[k] = res[k]
     if "fp64" in res:

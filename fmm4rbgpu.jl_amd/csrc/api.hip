@@ -248,23 +248,32 @@ static void fork_upward(Ctx &c) {
     FMM_HIP(hipEventRecord(c.evu[2], s3));
 }
 
+// refit: keep the tree and lists of the last completed call with the same N, N0
+// and eta (particle order, ranges, splits, lists); recompute p_avg, the bounding
+// boxes and the interpolation nodes for the new inputs.
 static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, double charge,
                       double lambda, int n, int64_t N0, double eta, int precision, double *E,
-                      double *B, fmm4rb_stats *stats) {
+                      double *B, fmm4rb_stats *stats, bool refit = false) {
     validate(N, n, N0, eta, precision);
     if (!std::isfinite(charge) || !std::isfinite(lambda) || lambda == 0.0)
         fail(FMM4RB_EINVAL, "charge must be finite and lambda finite and non-zero");
+    if (refit && !(c.tree_valid && c.tree_N == N && c.tree_N0 == N0 && c.tree_eta == eta))
+        fail(FMM4RB_EINVAL,
+             "no tree to refit: the context's last field or reverse-mode call must have "
+             "completed with the same N, N0 and eta");
+    if (!refit) c.tree_valid = false;
     prepare(c, N, N0, n, precision);
     hipStream_t st = c.stream;
     const double amp = (AMP_RE * charge) / lambda;  // update_particles_field.jl:142
     FMM_HIP(hipEventRecord(c.ev[0], st));
     stage_pavg(c, pos, mom);
     FMM_HIP(hipEventRecord(c.ev[1], st));
-    build_tree_device(c, pos, mom, N, N0);
+    if (refit) refit_tree_device(c, pos, mom);
+    else build_tree_device(c, pos, mom, N, N0);
     launch_node_geo(c);
     FMM_HIP(hipEventRecord(c.ev[2], st));
     fork_upward(c);
-    build_lists_device(c, eta);
+    if (!refit) build_lists_device(c, eta);
     FMM_HIP(hipEventRecord(c.ev[3], st));
     // the near field needs only the tree order and the lists: it runs on the
     // auxiliary stream while this one forms the far field, and L2P waits for it
@@ -305,6 +314,10 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
         FMM_HIP(hipMemcpyAsync(ls, tmp.p, 16, hipMemcpyDeviceToHost, st));
     }
     FMM_HIP(hipStreamSynchronize(st));
+    c.tree_valid = true;
+    c.tree_N = N;
+    c.tree_N0 = N0;
+    c.tree_eta = eta;
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->N = N;
@@ -340,6 +353,7 @@ static void run_vjp(Ctx &c, int64_t N, const double *pos, const double *mom, dou
     validate(N, n, N0, eta, precision);
     if (!std::isfinite(charge) || !std::isfinite(lambda) || lambda == 0.0)
         fail(FMM4RB_EINVAL, "charge must be finite and lambda finite and non-zero");
+    c.tree_valid = false;
     prepare(c, N, N0, n, precision);
     hipStream_t st = c.stream;
     const double amp = (AMP_RE * charge) / lambda;
@@ -358,6 +372,10 @@ static void run_vjp(Ctx &c, int64_t N, const double *pos, const double *mom, dou
     run_adjoint(c, Ebar, Bbar, amp, gpos, gmom);
     FMM_HIP(hipEventRecord(c.ev[6], st));
     FMM_HIP(hipStreamSynchronize(st));
+    c.tree_valid = true;
+    c.tree_N = N;
+    c.tree_N0 = N0;
+    c.tree_eta = eta;
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->N = N;
@@ -511,11 +529,10 @@ int fmm4rb_field_device(fmm4rb_ctx *ctx, int64_t N, const double *d_pos, const d
     FMM_API_END
 }
 
-int fmm4rb_field(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom, double charge,
-                 double lambda, int n, int64_t N0, double eta, int precision, double *E, double *B,
-                 fmm4rb_stats *stats) {
-    FMM_API_BEGIN
-    FMM_NEED(ctx && pos && mom && E && B, "NULL argument");
+// Host-pointer form: stage pos/mom, run, copy E/B back.
+static void field_host(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
+                       double charge, double lambda, int n, int64_t N0, double eta, int precision,
+                       double *E, double *B, fmm4rb_stats *stats, bool refit) {
     validate(N, n, N0, eta, precision);
     set_device(ctx);
     hipStream_t st = ctx->c.stream;
@@ -527,10 +544,39 @@ int fmm4rb_field(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mo
     FMM_HIP(hipMemcpyAsync(ctx->h_pos.p, pos, bytes, hipMemcpyHostToDevice, st));
     FMM_HIP(hipMemcpyAsync(ctx->h_mom.p, mom, bytes, hipMemcpyHostToDevice, st));
     run_field(ctx->c, N, ctx->h_pos.as<double>(), ctx->h_mom.as<double>(), charge, lambda, n, N0,
-              eta, precision, ctx->h_E.as<double>(), ctx->h_B.as<double>(), stats);
+              eta, precision, ctx->h_E.as<double>(), ctx->h_B.as<double>(), stats, refit);
     FMM_HIP(hipMemcpyAsync(E, ctx->h_E.p, bytes, hipMemcpyDeviceToHost, st));
     FMM_HIP(hipMemcpyAsync(B, ctx->h_B.p, bytes, hipMemcpyDeviceToHost, st));
     FMM_HIP(hipStreamSynchronize(st));
+}
+
+int fmm4rb_field(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom, double charge,
+                 double lambda, int n, int64_t N0, double eta, int precision, double *E, double *B,
+                 fmm4rb_stats *stats) {
+    FMM_API_BEGIN
+    FMM_NEED(ctx && pos && mom && E && B, "NULL argument");
+    field_host(ctx, N, pos, mom, charge, lambda, n, N0, eta, precision, E, B, stats, false);
+    FMM_API_END
+}
+
+int fmm4rb_field_refit(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
+                       double charge, double lambda, int n, int64_t N0, double eta, int precision,
+                       double *E, double *B, fmm4rb_stats *stats) {
+    FMM_API_BEGIN
+    FMM_NEED(ctx && pos && mom && E && B, "NULL argument");
+    field_host(ctx, N, pos, mom, charge, lambda, n, N0, eta, precision, E, B, stats, true);
+    FMM_API_END
+}
+
+int fmm4rb_field_device_refit(fmm4rb_ctx *ctx, int64_t N, const double *d_pos,
+                              const double *d_mom, double charge, double lambda, int n,
+                              int64_t N0, double eta, int precision, double *d_E, double *d_B,
+                              fmm4rb_stats *stats) {
+    FMM_API_BEGIN
+    FMM_NEED(ctx && d_pos && d_mom && d_E && d_B, "NULL argument");
+    set_device(ctx);
+    run_field(ctx->c, N, d_pos, d_mom, charge, lambda, n, N0, eta, precision, d_E, d_B, stats,
+              true);
     FMM_API_END
 }
 
@@ -651,6 +697,7 @@ static void hook_tree(fmm4rb_ctx *ctx, int64_t N, const double *pos, const doubl
     validate(N, n, N0, 0.0, FMM4RB_FP64);
     set_device(ctx);
     Ctx &c = ctx->c;
+    c.tree_valid = false;
     hipStream_t st = c.stream;
     const size_t bytes = 3 * (size_t)N * sizeof(double);
     ctx->h_pos.ensure(bytes);
@@ -778,6 +825,7 @@ int fmm4rb_op_m2l(fmm4rb_ctx *ctx, int n, int precision, int64_t nc, const doubl
     FMM_NEED(precision == FMM4RB_FP64 || precision == FMM4RB_MIXED, "bad precision");
     set_device(ctx);
     Ctx &c = ctx->c;
+    c.tree_valid = false;
     hipStream_t st = c.stream;
     hook_shape(c, nc, nullptr, nullptr, 0);
     c.n = n;
@@ -834,6 +882,7 @@ int fmm4rb_op_p2p(fmm4rb_ctx *ctx, int precision, int64_t N, const double *pos, 
         FMM_NEED(parindices[i] >= 0 && parindices[i] < N, "parindices out of range");
     set_device(ctx);
     Ctx &c = ctx->c;
+    c.tree_valid = false;
     hipStream_t st = c.stream;
     hook_shape(c, nc, lo, hi, N);
     c.precision = precision;

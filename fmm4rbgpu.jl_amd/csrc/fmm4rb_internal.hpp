@@ -110,6 +110,8 @@ void scan_rows(uint32_t *rows, int nrows, int len, uint32_t *tot, hipStream_t st
 void launch_pavg(const double *pos, const double *mom, int64_t N, double *part, double *pavg4,
                  int *bad, hipStream_t st);
 void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, int64_t N0);
+// New positions on the existing tree: tree-ordered particles and every bounding box.
+void refit_tree_device(Ctx &c, const double *pos, const double *mom);
 
 // ---- traversal (traverse.hip)
 void build_lists_device(Ctx &c, double eta);
@@ -162,6 +164,11 @@ struct Ctx {
     hipStream_t near_stream() const { return overlap ? aux : stream; }
     hipStream_t third_stream() const { return overlap ? aux2 : stream; }
     bool debug = false;
+    // the tree and lists on the device belong to a completed field or reverse-mode
+    // call with these N, N0, eta (the refit entry points reuse them)
+    bool tree_valid = false;
+    int64_t tree_N = 0, tree_N0 = 0;
+    double tree_eta = -1.0;
 
     // parameters of the current call
     int64_t N = 0, N0 = 0;

@@ -452,6 +452,57 @@ __global__ void k_gather(int64_t N, const int32_t *__restrict__ parind,
         reinterpret_cast<float4 *>(sp4f)[i] = make_float4((float)g, (float)m0, (float)m1, (float)m2);
 }
 
+// ------------------------------------------------------------------ refit
+// Bounding boxes of an existing tree for new positions: the particle order,
+// cluster ranges and split structure stay.  A leaf's box is the min/max of its
+// particles' coordinates (clusters.jl:51-70; one wave per leaf over the
+// tree-ordered copy), a split node's box the union of its two children's (they
+// partition its particles), level by level upward.  Min and max are exact, so
+// unchanged positions give the boxes of the full build.
+__global__ __launch_bounds__(64) void k_refit_leaves(const int32_t *__restrict__ leaves,
+                                                     const int32_t *__restrict__ lo,
+                                                     const int32_t *__restrict__ cnt,
+                                                     const double *__restrict__ sp8,
+                                                     double *__restrict__ bbox) {
+    const int32_t k = leaves[blockIdx.x];
+    const int64_t a = lo[k];
+    const int c = cnt[k];
+    double mn[3], mx[3];
+    for (int d = 0; d < 3; ++d) {
+        mn[d] = __builtin_huge_val();
+        mx[d] = -__builtin_huge_val();
+    }
+    for (int p = threadIdx.x; p < c; p += 64) {
+        const double *x = sp8 + 8 * (a + p);
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin(mn[d], x[d]);
+            mx[d] = fmax(mx[d], x[d]);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin(mn[d], __shfl_xor(mn[d], off));
+            mx[d] = fmax(mx[d], __shfl_xor(mx[d], off));
+        }
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) {
+            bbox[6 * (int64_t)k + d] = mn[d] + 0.0;  // -0.0 -> +0.0, as the build
+            bbox[6 * (int64_t)k + 3 + d] = mx[d] + 0.0;
+        }
+}
+
+__global__ void k_refit_parents(int nnodes, const int32_t *__restrict__ nodes,
+                                double *__restrict__ bbox) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nnodes) return;
+    const int64_t k = nodes[t];
+    const double *l = bbox + 6 * (2 * k + 1), *r = bbox + 6 * (2 * k + 2);
+    for (int d = 0; d < 3; ++d) {
+        bbox[6 * k + d] = fmin(l[d], r[d]);
+        bbox[6 * k + 3 + d] = fmax(l[3 + d], r[3 + d]);
+    }
+}
+
 // ------------------------------------------------------------- host side
 static inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
@@ -530,6 +581,28 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
                        pos, mom, c.d_sp8.as<double>(),
                        c.precision == FMM4RB_MIXED ? c.d_sp8f.as<float>() : (float *)nullptr);
     FMM_LAUNCH_CHECK();
+}
+
+void refit_tree_device(Ctx &c, const double *pos, const double *mom) {
+    const Shape &S = c.shape;
+    hipStream_t st = c.stream;
+    const int64_t N = c.N;
+    hipLaunchKernelGGL(k_gather, dim3(nb(N, 256)), dim3(256), 0, st, N, c.d_parind.as<int32_t>(),
+                       pos, mom, c.d_sp8.as<double>(),
+                       c.precision == FMM4RB_MIXED ? c.d_sp8f.as<float>() : (float *)nullptr);
+    FMM_LAUNCH_CHECK();
+    const int nleaf = (int)S.leaves.size();
+    hipLaunchKernelGGL(k_refit_leaves, dim3(nleaf), dim3(64), 0, st, c.d_leaves.as<int32_t>(),
+                       c.d_lo.as<int32_t>(), c.d_cnt.as<int32_t>(), c.d_sp8.as<double>(),
+                       c.d_bbox.as<double>());
+    FMM_LAUNCH_CHECK();
+    for (int l = S.ml - 1; l >= 0; --l) {  // split nodes of level l: split_off[l] .. split_off[l+1]
+        const int64_t b = S.split_off[l], e = S.split_off[l + 1];
+        if (e == b) continue;
+        hipLaunchKernelGGL(k_refit_parents, dim3(nb(e - b, 128)), dim3(128), 0, st, (int)(e - b),
+                           c.d_lvl_split.as<int32_t>() + b, c.d_bbox.as<double>());
+        FMM_LAUNCH_CHECK();
+    }
 }
 
 }  // namespace fmm4rb

@@ -34,7 +34,7 @@ EXPORTS = (
     "fmm4rb_set_stream", "fmm4rb_set_overlap", "fmm4rb_field", "fmm4rb_field_device",
     "fmm4rb_bruteforce", "fmm4rb_tree", "fmm4rb_lists", "fmm4rb_op_m2l", "fmm4rb_op_p2p",
     "fmm4rb_set_debug", "fmm4rb_debug_fetch", "fmm4rb_field_vjp", "fmm4rb_field_vjp_device",
-    "fmm4rb_bruteforce_vjp",
+    "fmm4rb_bruteforce_vjp", "fmm4rb_field_refit", "fmm4rb_field_device_refit",
 )
 
 
@@ -98,6 +98,8 @@ def lib():
                                    C.c_double, C.c_int, _dp, _dp, C.POINTER(Stats)]
         L.fmm4rb_field_device.argtypes = [_vp, _i64, _vp, _vp, C.c_double, C.c_double, C.c_int,
                                           _i64, C.c_double, C.c_int, _vp, _vp, C.POINTER(Stats)]
+        L.fmm4rb_field_refit.argtypes = L.fmm4rb_field.argtypes
+        L.fmm4rb_field_device_refit.argtypes = L.fmm4rb_field_device.argtypes
         L.fmm4rb_bruteforce.argtypes = [_vp, _i64, _dp, _dp, C.c_double, C.c_double, _i64, _vp,
                                         _dp, _dp]
         L.fmm4rb_tree.argtypes = [_vp, _i64, _dp, _dp, _i64, _ip, _ip, _ip, _dp, _ip, _dp]
@@ -196,6 +198,28 @@ class Context:
         st = Stats()
         _check(lib().fmm4rb_field_device(self._h, N, pos_ptr, mom_ptr, charge, lam, n, N0, eta,
                                          _PREC[precision], E_ptr, B_ptr, C.byref(st)))
+        return st
+
+    def field_refit(self, pos, mom, charge=-1.0, lam=1.0, n=4, N0=125, eta=0.5,
+                    precision="fp64"):
+        """Like field() on the tree and lists of this context's last completed field or
+        reverse-mode call with the same N, N0, eta (tracking steps): bounding boxes,
+        p_avg and expansions are recomputed, the partition and lists are kept."""
+        pos, mom = _f64(pos), _f64(mom)
+        N = pos.shape[0]
+        E, B = np.empty((N, 3)), np.empty((N, 3))
+        st = Stats()
+        _check(lib().fmm4rb_field_refit(self._h, N, pos, mom, charge, lam, n, N0, eta,
+                                        _PREC[precision], E, B, C.byref(st)))
+        return E, B, st
+
+    def field_device_refit(self, N, pos_ptr, mom_ptr, E_ptr, B_ptr, charge=-1.0, lam=1.0, n=4,
+                           N0=125, eta=0.5, precision="fp64"):
+        """field_refit() with device pointers -> stats."""
+        st = Stats()
+        _check(lib().fmm4rb_field_device_refit(self._h, N, pos_ptr, mom_ptr, charge, lam, n, N0,
+                                               eta, _PREC[precision], E_ptr, B_ptr,
+                                               C.byref(st)))
         return st
 
     def field_vjp(self, pos, mom, Ebar, Bbar, charge=-1.0, lam=1.0, n=4, N0=125, eta=0.5,

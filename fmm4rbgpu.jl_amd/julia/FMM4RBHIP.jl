@@ -52,27 +52,38 @@ precision_code(p::Symbol) = p === :fp64 ? FP64 : p === :mixed ? MIXED :
                             throw(ArgumentError("precision must be :fp64 or :mixed"))
 
 """
-    update_fields!(positions, momenta, efields, bfields, charge, alg::FMMHIP; lambda)
+    update_fields!(positions, momenta, efields, bfields, charge, alg::FMMHIP; lambda,
+                   refit = false)
 
 Vectors of SVector{3,Float64} (isbits, contiguous): the fields are overwritten,
-as FMMGPU does (src/update_particles_field.jl:74-75,146-147).
+as FMMGPU does (src/update_particles_field.jl:74-75,146-147).  `refit = true`
+(tracking steps) keeps the tree and interaction lists of the context's last call
+with the same N, N0 and eta and recomputes bounding boxes, p_avg and expansions
+(`fmm4rb_field_refit`); rebuild with `refit = false` as particles drift.
 """
 function update_fields!(positions::AbstractVector, momenta::AbstractVector,
                         efields::AbstractVector, bfields::AbstractVector,
-                        charge::Real, alg::FMMHIP; lambda, ctx::Context = default_context())
+                        charge::Real, alg::FMMHIP; lambda, ctx::Context = default_context(),
+                        refit::Bool = false)
     N = length(positions)
     @assert length(momenta) == N && length(efields) == N && length(bfields) == N
     pos = reinterpret(Float64, positions)
     mom = reinterpret(Float64, momenta)
     E = reinterpret(Float64, efields)
     B = reinterpret(Float64, bfields)
+    args = (ctx.ptr, N, pointer(pos), pointer(mom), Float64(charge), Float64(lambda),
+            Cint(alg.n), Int64(alg.N0), Float64(alg.eta), precision_code(alg.precision),
+            pointer(E), pointer(B), C_NULL)
     GC.@preserve pos mom E B begin
-        check(ccall((:fmm4rb_field, libfmm4rb), Cint,
-                    (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Cint, Int64,
-                     Float64, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
-                    ctx.ptr, N, pointer(pos), pointer(mom), Float64(charge), Float64(lambda),
-                    Cint(alg.n), Int64(alg.N0), Float64(alg.eta), precision_code(alg.precision),
-                    pointer(E), pointer(B), C_NULL))
+        if refit
+            check(ccall((:fmm4rb_field_refit, libfmm4rb), Cint,
+                        (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Cint,
+                         Int64, Float64, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}), args...))
+        else
+            check(ccall((:fmm4rb_field, libfmm4rb), Cint,
+                        (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Cint,
+                         Int64, Float64, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}), args...))
+        end
     end
     return nothing
 end

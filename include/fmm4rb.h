@@ -87,6 +87,27 @@ int fmm4rb_field_device(fmm4rb_ctx *ctx, int64_t N, const double *d_pos, const d
                         int precision, double *d_E, double *d_B, fmm4rb_stats *stats);
 
 /*
+ * Tracking steps (no reference counterpart: the reference rebuilds the tree on
+ * every call, update_particles_field.jl:96-104).  The field for new positions and
+ * momenta on the tree and interaction lists of this context's last completed
+ * fmm4rb_field* or fmm4rb_field_vjp* call, which must have had the same N, N0 and
+ * eta (else FMM4RB_EINVAL; n and precision may differ).  Kept: the particle
+ * order, cluster ranges, splits and both lists.  Recomputed: p_avg, every
+ * cluster's bounding box (min/max over its particles, clusters.jl:51-70), the
+ * interpolation nodes and all expansions.  With the inputs of that call the
+ * result is bit-identical to it; as particles drift the partition goes stale
+ * and the error grows until the caller rebuilds with fmm4rb_field*.  Skips the
+ * tree build and the traversal (about 7 of 34 ms at N = 2^22 mixed).
+ */
+int fmm4rb_field_refit(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
+                       double charge, double lambda, int n, int64_t N0, double eta,
+                       int precision, double *E, double *B, fmm4rb_stats *stats);
+int fmm4rb_field_device_refit(fmm4rb_ctx *ctx, int64_t N, const double *d_pos,
+                              const double *d_mom, double charge, double lambda, int n,
+                              int64_t N0, double eta, int precision, double *d_E, double *d_B,
+                              fmm4rb_stats *stats);
+
+/*
  * Reverse mode (no reference counterpart): the vector-Jacobian product of the
  * field map at fixed tree and lists.  Given cotangents Ebar, Bbar (3N each,
  * of the fields as fmm4rb_field returns them), writes gpos = sum Ebar.dE/dpos
