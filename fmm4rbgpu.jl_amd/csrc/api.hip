@@ -272,9 +272,19 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
     else build_tree_device(c, pos, mom, N, N0);
     launch_node_geo(c);
     FMM_HIP(hipEventRecord(c.ev[2], st));
-    fork_upward(c);
-    if (!refit) build_lists_device(c, eta);
-    FMM_HIP(hipEventRecord(c.ev[3], st));
+    if (refit) {
+        // no list build to run beside: the upward pass goes first on this stream, so
+        // that M2L starts together with P2P and the two share the GPU
+        FMM_HIP(hipEventRecord(c.ev[3], st));
+        FMM_HIP(hipEventRecord(c.evu[1], st));
+        launch_upward(c, st);
+        if (c.precision == FMM4RB_MIXED) launch_mp_to_f32(c, st);
+        FMM_HIP(hipEventRecord(c.evu[2], st));
+    } else {
+        fork_upward(c);
+        build_lists_device(c, eta);
+        FMM_HIP(hipEventRecord(c.ev[3], st));
+    }
     // the near field needs only the tree order and the lists: it runs on the
     // auxiliary stream while this one forms the far field, and L2P waits for it
     FMM_HIP(hipEventRecord(c.evx[0], st));
