@@ -125,21 +125,25 @@ __global__ void k_make_keys(const double *__restrict__ pos, int64_t N, uint64_t 
     seg[i] = 0;
 }
 
-// rank[d][perm_d[r]] = r
-__global__ void k_ranks(int64_t N, const uint32_t *__restrict__ perm, uint32_t *__restrict__ rank) {
+// rank4[p] = (r_x, r_y, r_z, -) with perm_d[r_d] = p: a particle's three ranks in
+// one 16-byte word, so that the lists below gather each particle once per list
+__global__ void k_ranks(int64_t N, const uint32_t *__restrict__ perm, uint32_t *__restrict__ rank4) {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= N) return;
-    for (int d = 0; d < 3; ++d) rank[d * N + perm[d * N + r]] = (uint32_t)r;
+    for (int d = 0; d < 3; ++d) rank4[4 * (int64_t)perm[d * N + r] + d] = (uint32_t)r;
 }
 
-// Level-0 lists: rk[(3 d + e) N + r] = r_e of the particle at rank r on axis d.
+// Level-0 lists: rk[(3 d + e) N + r] = r_e of the particle at rank r on axis d
+// (its own rank r_d = r).
 __global__ void k_lists_init(int64_t N, const uint32_t *__restrict__ perm,
-                             const uint32_t *__restrict__ rank, uint32_t *__restrict__ rk) {
+                             const uint4 *__restrict__ rank4, uint32_t *__restrict__ rk) {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= N) return;
     for (int d = 0; d < 3; ++d) {
-        const uint32_t p = perm[d * N + r];
-        for (int e = 0; e < 3; ++e) rk[(3 * d + e) * N + r] = e == d ? (uint32_t)r : rank[e * N + p];
+        const uint4 t = rank4[perm[d * N + r]];
+        rk[(3 * d + 0) * N + r] = t.x;
+        rk[(3 * d + 1) * N + r] = t.y;
+        rk[(3 * d + 2) * N + r] = t.z;
     }
 }
 
@@ -235,12 +239,32 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(
     const int d = blockIdx.y;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t base = (int64_t)blockIdx.x * PT_TILE;
-    uint64_t bal[PT_ITEMS];
+    const uint32_t *r0 = rk + (int64_t)(3 * d) * N;
+    // every entry's segment and rank triple is loaded once, up front, and kept in
+    // registers through the side decision and the move
+    int32_t kk[PT_ITEMS];
+    uint32_t v[PT_ITEMS][3];
 #pragma unroll
     for (int j = 0; j < PT_ITEMS; ++j) {
         const int64_t i = base + j * PT_THREADS + threadIdx.x;
+        kk[j] = -1;
+        v[j][0] = v[j][1] = v[j][2] = 0u;
+        if (i < N) {
+            kk[j] = seg[i];
+#pragma unroll
+            for (int e = 0; e < 3; ++e) v[j][e] = r0[e * N + i];
+        }
+    }
+    uint64_t bal[PT_ITEMS];
+#pragma unroll
+    for (int j = 0; j < PT_ITEMS; ++j) {
+        const int32_t k = kk[j];
         uint32_t f = 0;
-        if (i < N) f = part_side(N, l, d, i, seg[i], flags, axis, pivot, rk);
+        if (k >= 0 && splitting(k, l, flags)) {  // as part_side, from the loaded triple
+            const int a = axis[k];
+            const uint32_t ra = a == 0 ? v[j][0] : (a == 1 ? v[j][1] : v[j][2]);
+            f = ra < pivot[k] ? 1u : 0u;
+        }
         bal[j] = __ballot(f);
         if (lane == 0) s[j][w] = (uint32_t)__popcll(bal[j]);
     }
@@ -251,12 +275,12 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(
     for (int j = 0; j < PT_ITEMS; ++j) {
         const int64_t i = base + j * PT_THREADS + threadIdx.x;
         uint32_t before = 0, total = 0;
-        for (int v = 0; v < PT_WAVES; ++v) {
-            before += v < w ? s[j][v] : 0u;
-            total += s[j][v];
+        for (int u = 0; u < PT_WAVES; ++u) {
+            before += u < w ? s[j][u] : 0u;
+            total += s[j][u];
         }
         if (i < N) {
-            const int32_t k = seg[i];
+            const int32_t k = kk[j];
             int64_t np = i;
             if (splitting(k, l, flags)) {
                 const int64_t lft = run + before + (uint32_t)__popcll(bal[j] & below) - leftbase[k];
@@ -267,7 +291,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(
                 seg_out[i] = k;
             }
 #pragma unroll
-            for (int e = 0; e < 3; ++e) rk_out[(3 * d + e) * N + np] = rk[(3 * d + e) * N + i];
+            for (int e = 0; e < 3; ++e) rk_out[(3 * d + e) * N + np] = v[j][e];
         }
         run += total;
     }
@@ -520,11 +544,13 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
     bool alt = radix_sort(keys, keys_alt, vals, vals_alt, N, 3, 0, 64,
                           c.d_rs_scratch.as<uint32_t>(), st);
     const uint32_t *perm = alt ? vals_alt : vals;
-    uint32_t *rank = alt ? vals : vals_alt;
-    hipLaunchKernelGGL(k_ranks, dim3(nb(N, 256)), dim3(256), 0, st, N, perm, rank);
+    // the sorted keys are not needed any more: their buffer (24 B per particle) holds
+    // the 16-byte rank words
+    uint32_t *rank4 = reinterpret_cast<uint32_t *>(keys);
+    hipLaunchKernelGGL(k_ranks, dim3(nb(N, 256)), dim3(256), 0, st, N, perm, rank4);
     FMM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_lists_init, dim3(nb(N, 256)), dim3(256), 0, st, N, perm,
-                       (const uint32_t *)rank, rk);
+                       reinterpret_cast<const uint4 *>(rank4), rk);
     FMM_LAUNCH_CHECK();
 
     const int nblk = (int)nb(N, PT_TILE);
