@@ -78,15 +78,20 @@ __global__ __launch_bounds__(RS_THREADS) void k_scan_rows(uint32_t *__restrict__
     if (threadIdx.x == 0) tot[row] = carry;
 }
 
+// Stable in-tile ranking: each wave owns RS_ITEMS x 64 consecutive keys of the tile
+// and ranks them 64 at a time in order, with per-wave digit counters in LDS (the
+// wave's own LDS accesses are in order, so no block barrier is needed between its
+// steps); one barrier and a per-digit prefix over the waves then give every key's
+// slot in the digit-ordered tile.
 template <bool HASVAL>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
     uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ hist, int nblk,
     const uint32_t *__restrict__ dtot) {
+    constexpr int RS_WAVES = RS_THREADS / 64;
     __shared__ uint32_t doff[RS_BINS];  // global position of this tile's first key of each digit
     __shared__ uint32_t lst[RS_BINS];   // local position of the same key within the tile
-    __shared__ uint32_t run[RS_BINS];
-    __shared__ uint32_t wcnt[RS_THREADS / 64][RS_BINS];
+    __shared__ uint32_t wcnt[RS_WAVES][RS_BINS];  // per-wave counts, then offsets over waves
     __shared__ uint32_t s[RS_THREADS];
     __shared__ uint64_t sk[RS_TILE];
     __shared__ uint32_t sv[HASVAL ? RS_TILE : 1];
@@ -104,19 +109,27 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     doff[tid] = incl - tv + pre;
     const uint32_t lincl = block_incl_scan256(cnt, s);
     lst[tid] = lincl - cnt;
-    run[tid] = 0;
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) wcnt[w][tid] = 0;
     __syncthreads();
 
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t wbase = base + (int64_t)wave * (RS_ITEMS * 64);
     const int tn = (int)min<int64_t>(RS_TILE, n - base);
+    uint64_t key[RS_ITEMS];
+    uint32_t val[RS_ITEMS], rnk[RS_ITEMS];
+#pragma unroll
     for (int r = 0; r < RS_ITEMS; ++r) {
-        const int64_t i = base + r * RS_THREADS + tid;
+        const int64_t i = wbase + r * 64 + lane;
         const bool valid = i < n;
-        const uint64_t key = valid ? kin[so + i] : 0ull;
-        uint32_t val = 0;
-        if (HASVAL && valid) val = vin[so + i];
-        const uint32_t dg = (uint32_t)(key >> shift) & 0xFFu;
+        key[r] = valid ? kin[so + i] : 0ull;
+        val[r] = (HASVAL && valid) ? vin[so + i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const bool valid = wbase + r * 64 + lane < n;
+        const uint32_t dg = (uint32_t)(key[r] >> shift) & 0xFFu;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -124,30 +137,39 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
             const uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
         }
-        for (int j = tid; j < (RS_THREADS / 64) * RS_BINS; j += RS_THREADS) (&wcnt[0][0])[j] = 0;
-        __syncthreads();
-        if (valid && (peers >> lane) == 1ull) wcnt[wave][dg] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t off = run[dg] + (uint32_t)__popcll(peers & lt);
-            for (int w = 0; w < wave; ++w) off += wcnt[w][dg];
-            const uint32_t lp = lst[dg] + off;  // < tn
-            sk[lp] = key;
-            if (HASVAL) sv[lp] = val;
-        }
-        __syncthreads();
-        uint32_t add = 0;
-#pragma unroll
-        for (int w = 0; w < RS_THREADS / 64; ++w) add += wcnt[w][tid];
-        run[tid] += add;
-        __syncthreads();
+        // this wave's keys of digit dg before this step, then the lower lanes of this step
+        const uint32_t prior = valid ? wcnt[wave][dg] : 0u;
+        rnk[r] = prior + (uint32_t)__popcll(peers & lt);
+        if (valid && (peers & lt) == 0ull) wcnt[wave][dg] = prior + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
     }
+    __syncthreads();
+    {  // per digit: exclusive prefix of the wave counts
+        uint32_t acc = 0;
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; ++w) {
+            const uint32_t c = wcnt[w][tid];
+            wcnt[w][tid] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        if (wbase + r * 64 + lane < n) {
+            const uint32_t dg = (uint32_t)(key[r] >> shift) & 0xFFu;
+            const uint32_t lp = lst[dg] + wcnt[wave][dg] + rnk[r];  // < tn
+            sk[lp] = key[r];
+            if (HASVAL) sv[lp] = val[r];
+        }
+    }
+    __syncthreads();
     // the tile in digit order: consecutive slots of one digit go to consecutive positions
     for (int j = tid; j < tn; j += RS_THREADS) {
-        const uint64_t key = sk[j];
-        const uint32_t dg = (uint32_t)(key >> shift) & 0xFFu;
+        const uint64_t k = sk[j];
+        const uint32_t dg = (uint32_t)(k >> shift) & 0xFFu;
         const uint32_t pos = doff[dg] + (uint32_t)j - lst[dg];
-        kout[so + pos] = key;
+        kout[so + pos] = k;
         if (HASVAL) vout[so + pos] = sv[j];
     }
 }
