@@ -203,6 +203,31 @@ def main():
         res["adjoint_mixed"]["vs_fp64"] = {
             "gpos": W.rel_l2(grads["mixed"][0], grads["fp64"][0]),
             "gmom": W.rel_l2(grads["mixed"][1], grads["fp64"][1])}
+        # the rrule's pullback: after a forward call on the same inputs, the reverse
+        # mode reuses its final local expansions (tree and lists still recomputed)
+        for prec in ("fp64", "mixed"):
+            gpos, gmom = torch.empty_like(dpos), torch.empty_like(dpos)
+            Ef, Bf = torch.empty_like(dpos), torch.empty_like(dpos)
+            ctx.field_device(a.N, dpos.data_ptr(), dmom.data_ptr(), Ef.data_ptr(), Bf.data_ptr(),
+                             -1.0, lam, a.n, a.N0, a.eta, prec)
+            call = lambda: ctx.field_pullback_device(a.N, dpos.data_ptr(), dmom.data_ptr(),
+                                                     Eb.data_ptr(), Bb.data_ptr(),
+                                                     gpos.data_ptr(), gmom.data_ptr(), -1.0, lam,
+                                                     a.n, a.N0, a.eta, prec)
+            for _ in range(a.warmup):
+                call()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sp = [call() for _ in range(a.steps)]
+            torch.cuda.synchronize()
+            ms_p = (time.perf_counter() - t0) / a.steps
+            g = (gpos.cpu().numpy(), gmom.cpu().numpy())
+            res["pullback_" + prec] = {
+                "ms_per_step": ms_p * 1e3, "ratio_to_fp64_forward": ms_p / ms_d,
+                "stages_ms": stage_avg(sp),
+                "vs_vjp": {"gpos": W.rel_l2(g[0], grads[prec][0]),
+                           "gmom": W.rel_l2(g[1], grads[prec][1])}}
+            del Ef, Bf
     res["library_device_bytes"]["after_reverse_mode"] = lib_bytes()
     # host-pointer entry point (fmm4rb_field): PCIe copies of pos/mom in and E/B out
     # included; reported beside the device-resident value, never as the value
@@ -241,8 +266,8 @@ def main():
         "stages_ms_isolated": stage_avg(iso_m),
         "roofline": roofline(s, stage_avg(iso_m), a.n, "mixed", a.N),
     }
-    for k in ("adjoint_fp64", "adjoint_mixed", "tracking_refit_mixed", "host_pointers_mixed",
-              "library_device_bytes"):
+    for k in ("adjoint_fp64", "adjoint_mixed", "pullback_fp64", "pullback_mixed",
+              "tracking_refit_mixed", "host_pointers_mixed", "library_device_bytes"):
         if k in res:
             out[k] = res[k]
     if "fp64" in res:

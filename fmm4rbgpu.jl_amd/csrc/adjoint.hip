@@ -1421,7 +1421,7 @@ static inline unsigned nbk(int64_t n, int t) { return (unsigned)((n + t - 1) / t
 
 
 void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, double *gpos,
-                 double *gmom) {
+                 double *gmom, bool saved_locals) {
     const Shape &S = c.shape;
     hipStream_t st = c.stream;
     const int64_t N = c.N, nc = S.nc, NN = c.nn, np = c.np;
@@ -1520,21 +1520,30 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
         ADJ_NP_SWITCH(c.np, ADJ_L2LV);
         FMM_LAUNCH_CHECK();
     }
-    // 2. forward M2L fused with the M2L^T target side, then the forward L2L
-#define ADJ_M2LT(NP)                                                                               \
+    // 2. forward M2L fused with the M2L^T target side, then the forward L2L; with
+    //    saved_locals the final forward locals of the preceding field call are in
+    //    loc already and only the M2L^T target side runs
+#define ADJ_M2LT_F(NP, FWD)                                                                        \
     if (mixed)                                                                                     \
-        hipLaunchKernelGGL((k_adj_m2l_t<NP, true, true>), dim3((unsigned)nc),                      \
+        hipLaunchKernelGGL((k_adj_m2l_t<NP, FWD, true>), dim3((unsigned)nc),                       \
                            dim3(AShape<NP>::BLK), 0, st, c.d_m2l_off.as<int32_t>(), c.m2l_sorted,  \
                            c.sb, ncoord, mpv, lbar, c.d_pavg.as<double>(), xbar, pbar, loc, posT, \
                            ysp);                                                                   \
     else                                                                                           \
-        hipLaunchKernelGGL((k_adj_m2l_t<NP, true, false>), dim3((unsigned)nc),                     \
+        hipLaunchKernelGGL((k_adj_m2l_t<NP, FWD, false>), dim3((unsigned)nc),                      \
                            dim3(AShape<NP>::BLK), 0, st, c.d_m2l_off.as<int32_t>(), c.m2l_sorted,  \
                            c.sb, ncoord, mpv, lbar, c.d_pavg.as<double>(), xbar, pbar, loc, posT, \
                            ysp)
-    ADJ_NP_SWITCH(c.np, ADJ_M2LT);
-    FMM_LAUNCH_CHECK();
-    launch_l2l(c);
+#define ADJ_M2LT(NP) ADJ_M2LT_F(NP, true)
+#define ADJ_M2LT_SAVED(NP) ADJ_M2LT_F(NP, false)
+    if (saved_locals) {
+        ADJ_NP_SWITCH(c.np, ADJ_M2LT_SAVED);
+        FMM_LAUNCH_CHECK();
+    } else {
+        ADJ_NP_SWITCH(c.np, ADJ_M2LT);
+        FMM_LAUNCH_CHECK();
+        launch_l2l(c);
+    }
     FMM_HIP(hipEventRecord(c.ev[8], st));
     // 3. geometry of L2P^T and L2L^T (needs the final forward locals; latency-bound
     //    launches): on the third stream, beside the M2L^T source side (4.), which

@@ -151,8 +151,8 @@ static void prepare(Ctx &c, int64_t N, int64_t N0, int n, int precision) {
     c.sb = S.ml + 1;
     const size_t nc = (size_t)S.nc, NN = (size_t)c.nn, n3 = 3 * (size_t)N;
     const int64_t nleaf = (int64_t)S.leaves.size();
-    c.d_part.ensure(3 * 1024 * sizeof(double));
-    c.d_pavg.ensure(4 * sizeof(double));
+    c.d_part.ensure(4 * 1024 * sizeof(double));  // 3 sums and a digest word per block
+    c.d_pavg.ensure(5 * sizeof(double));         // p_avg, g_avg, input digest
     c.d_bad.ensure(sizeof(int));
     c.d_keys.ensure(n3 * 8);
     c.d_keys_alt.ensure(n3 * 8);
@@ -231,6 +231,8 @@ static void stage_pavg(Ctx &c, const double *pos, const double *mom) {
     int bad = 0;
     FMM_HIP(hipMemcpyAsync(&bad, c.d_bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
     FMM_HIP(hipMemcpyAsync(c.pavg_host, c.d_pavg.p, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(&c.in_digest, c.d_pavg.as<double>() + 4, sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, st));
     FMM_HIP(hipStreamSynchronize(st));
     if (bad) fail(FMM4RB_EINVAL, "positions or momenta contain NaN or Inf");
 }
@@ -262,6 +264,7 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
              "no tree to refit: the context's last field or reverse-mode call must have "
              "completed with the same N, N0 and eta");
     if (!refit) c.tree_valid = false;
+    c.loc_valid = false;
     prepare(c, N, N0, n, precision);
     hipStream_t st = c.stream;
     const double amp = (AMP_RE * charge) / lambda;  // update_particles_field.jl:142
@@ -328,6 +331,14 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
     c.tree_N = N;
     c.tree_N0 = N0;
     c.tree_eta = eta;
+    // a refit's locals belong to the stale tree, not to the one a pullback rebuilds
+    c.loc_valid = !refit;
+    c.loc_digest = c.in_digest;
+    c.loc_N = N;
+    c.loc_N0 = N0;
+    c.loc_n = n;
+    c.loc_prec = precision;
+    c.loc_eta = eta;
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->N = N;
@@ -355,20 +366,31 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
     }
 }
 
-// Reverse mode: forward recompute of the expansions (no P2P / L2P), then the adjoint.
+// Reverse mode: tree, lists and upward pass rebuilt, then the adjoint.  The forward
+// M2L and L2L are recomputed inside it, except for the pullback (saved_locals),
+// which takes the final local expansions of the preceding field call on the same
+// inputs from loc (verified by parameters and the input digest).
 static void run_vjp(Ctx &c, int64_t N, const double *pos, const double *mom, double charge,
                     double lambda, int n, int64_t N0, double eta, int precision,
                     const double *Ebar, const double *Bbar, double *gpos, double *gmom,
-                    fmm4rb_stats *stats) {
+                    fmm4rb_stats *stats, bool saved_locals = false) {
     validate(N, n, N0, eta, precision);
     if (!std::isfinite(charge) || !std::isfinite(lambda) || lambda == 0.0)
         fail(FMM4RB_EINVAL, "charge must be finite and lambda finite and non-zero");
+    const char *nofwd =
+        "no matching forward call: the pullback needs this context's last call to be a "
+        "completed fmm4rb_field* with the same inputs, N, N0, n, eta and precision";
+    if (saved_locals && !(c.loc_valid && c.loc_N == N && c.loc_N0 == N0 && c.loc_n == n &&
+                          c.loc_prec == precision && c.loc_eta == eta))
+        fail(FMM4RB_EINVAL, nofwd);
+    if (!saved_locals) c.loc_valid = false;
     c.tree_valid = false;
     prepare(c, N, N0, n, precision);
     hipStream_t st = c.stream;
     const double amp = (AMP_RE * charge) / lambda;
     FMM_HIP(hipEventRecord(c.ev[0], st));
     stage_pavg(c, pos, mom);
+    if (saved_locals && c.in_digest != c.loc_digest) fail(FMM4RB_EINVAL, nofwd);
     FMM_HIP(hipEventRecord(c.ev[1], st));
     build_tree_device(c, pos, mom, N, N0);
     launch_node_geo(c);
@@ -379,7 +401,7 @@ static void run_vjp(Ctx &c, int64_t N, const double *pos, const double *mom, dou
     FMM_HIP(hipStreamWaitEvent(st, c.evu[2], 0));  // join the upward pass
     FMM_HIP(hipEventRecord(c.ev[4], st));
     FMM_HIP(hipEventRecord(c.ev[5], st));
-    run_adjoint(c, Ebar, Bbar, amp, gpos, gmom);
+    run_adjoint(c, Ebar, Bbar, amp, gpos, gmom, saved_locals);
     FMM_HIP(hipEventRecord(c.ev[6], st));
     FMM_HIP(hipStreamSynchronize(st));
     c.tree_valid = true;
@@ -602,12 +624,24 @@ int fmm4rb_field_vjp_device(fmm4rb_ctx *ctx, int64_t N, const double *d_pos, con
     FMM_API_END
 }
 
-int fmm4rb_field_vjp(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
+int fmm4rb_field_pullback_device(fmm4rb_ctx *ctx, int64_t N, const double *d_pos,
+                                 const double *d_mom, double charge, double lambda, int n,
+                                 int64_t N0, double eta, int precision, const double *d_Ebar,
+                                 const double *d_Bbar, double *d_gpos, double *d_gmom,
+                                 fmm4rb_stats *stats) {
+    FMM_API_BEGIN
+    FMM_NEED(ctx && d_pos && d_mom && d_Ebar && d_Bbar && d_gpos && d_gmom, "NULL argument");
+    set_device(ctx);
+    run_vjp(ctx->c, N, d_pos, d_mom, charge, lambda, n, N0, eta, precision, d_Ebar, d_Bbar,
+            d_gpos, d_gmom, stats, true);
+    FMM_API_END
+}
+
+// Host-pointer reverse mode: stage inputs and cotangents, run, copy gradients back.
+static void vjp_host(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
                      double charge, double lambda, int n, int64_t N0, double eta, int precision,
                      const double *Ebar, const double *Bbar, double *gpos, double *gmom,
-                     fmm4rb_stats *stats) {
-    FMM_API_BEGIN
-    FMM_NEED(ctx && pos && mom && Ebar && Bbar && gpos && gmom, "NULL argument");
+                     fmm4rb_stats *stats, bool saved_locals) {
     validate(N, n, N0, eta, precision);
     set_device(ctx);
     hipStream_t st = ctx->c.stream;
@@ -620,10 +654,31 @@ int fmm4rb_field_vjp(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double
     FMM_HIP(hipMemcpyAsync(ctx->h_B.p, Bbar, bytes, hipMemcpyHostToDevice, st));
     run_vjp(ctx->c, N, ctx->h_pos.as<double>(), ctx->h_mom.as<double>(), charge, lambda, n, N0,
             eta, precision, ctx->h_E.as<double>(), ctx->h_B.as<double>(),
-            ctx->h_gpos.as<double>(), ctx->h_gmom.as<double>(), stats);
+            ctx->h_gpos.as<double>(), ctx->h_gmom.as<double>(), stats, saved_locals);
     FMM_HIP(hipMemcpyAsync(gpos, ctx->h_gpos.p, bytes, hipMemcpyDeviceToHost, st));
     FMM_HIP(hipMemcpyAsync(gmom, ctx->h_gmom.p, bytes, hipMemcpyDeviceToHost, st));
     FMM_HIP(hipStreamSynchronize(st));
+}
+
+int fmm4rb_field_pullback(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
+                          double charge, double lambda, int n, int64_t N0, double eta,
+                          int precision, const double *Ebar, const double *Bbar, double *gpos,
+                          double *gmom, fmm4rb_stats *stats) {
+    FMM_API_BEGIN
+    FMM_NEED(ctx && pos && mom && Ebar && Bbar && gpos && gmom, "NULL argument");
+    vjp_host(ctx, N, pos, mom, charge, lambda, n, N0, eta, precision, Ebar, Bbar, gpos, gmom,
+             stats, true);
+    FMM_API_END
+}
+
+int fmm4rb_field_vjp(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
+                     double charge, double lambda, int n, int64_t N0, double eta, int precision,
+                     const double *Ebar, const double *Bbar, double *gpos, double *gmom,
+                     fmm4rb_stats *stats) {
+    FMM_API_BEGIN
+    FMM_NEED(ctx && pos && mom && Ebar && Bbar && gpos && gmom, "NULL argument");
+    vjp_host(ctx, N, pos, mom, charge, lambda, n, N0, eta, precision, Ebar, Bbar, gpos, gmom,
+             stats, false);
     FMM_API_END
 }
 
@@ -708,6 +763,7 @@ static void hook_tree(fmm4rb_ctx *ctx, int64_t N, const double *pos, const doubl
     set_device(ctx);
     Ctx &c = ctx->c;
     c.tree_valid = false;
+    c.loc_valid = false;
     hipStream_t st = c.stream;
     const size_t bytes = 3 * (size_t)N * sizeof(double);
     ctx->h_pos.ensure(bytes);
@@ -836,6 +892,7 @@ int fmm4rb_op_m2l(fmm4rb_ctx *ctx, int n, int precision, int64_t nc, const doubl
     set_device(ctx);
     Ctx &c = ctx->c;
     c.tree_valid = false;
+    c.loc_valid = false;
     hipStream_t st = c.stream;
     hook_shape(c, nc, nullptr, nullptr, 0);
     c.n = n;
@@ -893,6 +950,7 @@ int fmm4rb_op_p2p(fmm4rb_ctx *ctx, int precision, int64_t N, const double *pos, 
     set_device(ctx);
     Ctx &c = ctx->c;
     c.tree_valid = false;
+    c.loc_valid = false;
     hipStream_t st = c.stream;
     hook_shape(c, nc, lo, hi, N);
     c.precision = precision;

@@ -135,7 +135,7 @@ void launch_p2p(Ctx &c, const int32_t *off, const uint64_t *keys, int sb, hipStr
 
 // ---- adjoint (adjoint.hip, traverse.hip)
 void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, double *gpos,
-                 double *gmom);
+                 double *gmom, bool saved_locals);
 void transpose_lists(Ctx &c, const uint64_t *sorted, int64_t n, DevBuf &kT, DevBuf &kT_alt,
                      DevBuf &offT, uint64_t **sortedT, DevBuf *pos = nullptr);
 void launch_bruteforce_vjp(const double *pos, const double *mom, int64_t N, const double *ob,
@@ -169,6 +169,13 @@ struct Ctx {
     bool tree_valid = false;
     int64_t tree_N = 0, tree_N0 = 0;
     double tree_eta = -1.0;
+    // the final local expansions in d_loc belong to a completed field call with
+    // these parameters and inputs (digest of pos and mom): the pullback reuses them
+    bool loc_valid = false;
+    uint64_t loc_digest = 0, in_digest = 0;  // in_digest: the current call's inputs
+    int64_t loc_N = 0, loc_N0 = 0;
+    int loc_n = 0, loc_prec = -1;
+    double loc_eta = -1.0;
 
     // parameters of the current call
     int64_t N = 0, N0 = 0;

@@ -37,13 +37,24 @@ constexpr int PT_WAVES = PT_THREADS / 64;
 
 // ----------------------------------------------------------------- p_avg
 // Deterministic fixed-order sum of the momenta and a finiteness check of all
-// inputs.  update_particles_field.jl:77 (reduce(+, momenta)/N).
-__global__ __launch_bounds__(PAVG_THREADS) void k_pavg_partial(const double *__restrict__ pos,
-                                                               const double *__restrict__ mom,
-                                                               int64_t N, double *__restrict__ part,
-                                                               int *__restrict__ bad) {
+// inputs.  update_particles_field.jl:77 (reduce(+, momenta)/N).  The same pass
+// forms a 64-bit digest of the inputs, the wrapping sum over all 6N input words of
+// a hash of (bits, index): independent of the summation order, so deterministic,
+// and it tells whether a later call sees the inputs of an earlier one.
+__device__ inline uint64_t word_hash(uint64_t x, uint64_t idx) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ull * (idx + 1ull);  // splitmix64 finaliser
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(PAVG_THREADS) void k_pavg_partial(
+    const double *__restrict__ pos, const double *__restrict__ mom, int64_t N,
+    double *__restrict__ part, unsigned long long *__restrict__ hpart, int *__restrict__ bad) {
     __shared__ double s[3][PAVG_THREADS];
+    __shared__ unsigned long long sh[PAVG_THREADS];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    uint64_t h = 0;
     int b = 0;
     for (int64_t i = (int64_t)blockIdx.x * PAVG_THREADS + threadIdx.x; i < N;
          i += (int64_t)gridDim.x * PAVG_THREADS) {
@@ -55,32 +66,47 @@ __global__ __launch_bounds__(PAVG_THREADS) void k_pavg_partial(const double *__r
         if (!(isfinite(m0) && isfinite(m1) && isfinite(m2) && isfinite(x0) && isfinite(x1) &&
               isfinite(x2)))
             b = 1;
+        const double v[6] = {x0, x1, x2, m0, m1, m2};
+        for (int e = 0; e < 6; ++e)
+            h += word_hash((uint64_t)__double_as_longlong(v[e]), 6 * (uint64_t)i + e);
     }
     s[0][threadIdx.x] = a0;
     s[1][threadIdx.x] = a1;
     s[2][threadIdx.x] = a2;
+    sh[threadIdx.x] = h;
     for (int off = PAVG_THREADS / 2; off > 0; off >>= 1) {
         __syncthreads();
-        if (threadIdx.x < (unsigned)off)
+        if (threadIdx.x < (unsigned)off) {
             for (int d = 0; d < 3; ++d) s[d][threadIdx.x] += s[d][threadIdx.x + off];
+            sh[threadIdx.x] += sh[threadIdx.x + off];
+        }
     }
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         for (int d = 0; d < 3; ++d) part[3 * blockIdx.x + d] = s[d][0];
+        hpart[blockIdx.x] = sh[0];
+    }
     if (b) atomicOr(bad, 1);
 }
 
-__global__ __launch_bounds__(PAVG_THREADS) void k_pavg_final(const double *__restrict__ part,
-                                                             int nblk, int64_t N,
-                                                             double *__restrict__ pavg4) {
+__global__ __launch_bounds__(PAVG_THREADS) void k_pavg_final(
+    const double *__restrict__ part, const unsigned long long *__restrict__ hpart, int nblk,
+    int64_t N, double *__restrict__ pavg4, unsigned long long *__restrict__ digest) {
     __shared__ double s[3][PAVG_THREADS];
+    __shared__ unsigned long long sh[PAVG_THREADS];
     double a[3] = {0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < nblk; i += PAVG_THREADS)
+    uint64_t h = 0;
+    for (int i = threadIdx.x; i < nblk; i += PAVG_THREADS) {
         for (int d = 0; d < 3; ++d) a[d] += part[3 * i + d];
+        h += hpart[i];
+    }
     for (int d = 0; d < 3; ++d) s[d][threadIdx.x] = a[d];
+    sh[threadIdx.x] = h;
     for (int off = PAVG_THREADS / 2; off > 0; off >>= 1) {
         __syncthreads();
-        if (threadIdx.x < (unsigned)off)
+        if (threadIdx.x < (unsigned)off) {
             for (int d = 0; d < 3; ++d) s[d][threadIdx.x] += s[d][threadIdx.x + off];
+            sh[threadIdx.x] += sh[threadIdx.x + off];
+        }
     }
     if (threadIdx.x == 0) {
         double p0 = s[0][0] / (double)N, p1 = s[1][0] / (double)N, p2 = s[2][0] / (double)N;
@@ -88,16 +114,22 @@ __global__ __launch_bounds__(PAVG_THREADS) void k_pavg_final(const double *__res
         pavg4[1] = p1;
         pavg4[2] = p2;
         pavg4[3] = sqrt(1.0 + ((p0 * p0 + p1 * p1) + p2 * p2));  // update_particles_field.jl:78
+        digest[0] = sh[0];
     }
 }
 
+// part: 3 PAVG_MAXBLK doubles then PAVG_MAXBLK digest words; pavg4: 4 doubles then
+// the digest word
 void launch_pavg(const double *pos, const double *mom, int64_t N, double *part, double *pavg4,
                  int *bad, hipStream_t st) {
     int nblk = (int)std::min<int64_t>(PAVG_MAXBLK, (N + PAVG_THREADS - 1) / PAVG_THREADS);
+    unsigned long long *hpart = reinterpret_cast<unsigned long long *>(part + 3 * PAVG_MAXBLK);
+    unsigned long long *digest = reinterpret_cast<unsigned long long *>(pavg4 + 4);
     hipLaunchKernelGGL(k_pavg_partial, dim3(nblk), dim3(PAVG_THREADS), 0, st, pos, mom, N, part,
-                       bad);
+                       hpart, bad);
     FMM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_pavg_final, dim3(1), dim3(PAVG_THREADS), 0, st, part, nblk, N, pavg4);
+    hipLaunchKernelGGL(k_pavg_final, dim3(1), dim3(PAVG_THREADS), 0, st, part, hpart, nblk, N,
+                       pavg4, digest);
     FMM_LAUNCH_CHECK();
 }
 

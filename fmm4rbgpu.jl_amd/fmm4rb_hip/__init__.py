@@ -35,6 +35,7 @@ EXPORTS = (
     "fmm4rb_bruteforce", "fmm4rb_tree", "fmm4rb_lists", "fmm4rb_op_m2l", "fmm4rb_op_p2p",
     "fmm4rb_set_debug", "fmm4rb_debug_fetch", "fmm4rb_field_vjp", "fmm4rb_field_vjp_device",
     "fmm4rb_bruteforce_vjp", "fmm4rb_field_refit", "fmm4rb_field_device_refit",
+    "fmm4rb_field_pullback", "fmm4rb_field_pullback_device",
 )
 
 
@@ -115,6 +116,8 @@ def lib():
         L.fmm4rb_field_vjp_device.argtypes = [_vp, _i64, _vp, _vp, C.c_double, C.c_double, C.c_int,
                                               _i64, C.c_double, C.c_int, _vp, _vp, _vp, _vp,
                                               C.POINTER(Stats)]
+        L.fmm4rb_field_pullback.argtypes = L.fmm4rb_field_vjp.argtypes
+        L.fmm4rb_field_pullback_device.argtypes = L.fmm4rb_field_vjp_device.argtypes
         L.fmm4rb_bruteforce_vjp.argtypes = [_vp, _i64, _dp, _dp, C.c_double, C.c_double, _dp, _dp,
                                             _dp, _dp]
         for name in EXPORTS:
@@ -240,6 +243,27 @@ class Context:
         _check(lib().fmm4rb_field_vjp_device(self._h, N, pos_ptr, mom_ptr, charge, lam, n, N0,
                                              eta, _PREC[precision], Ebar_ptr, Bbar_ptr,
                                              gpos_ptr, gmom_ptr, C.byref(st)))
+        return st
+
+    def field_pullback(self, pos, mom, Ebar, Bbar, charge=-1.0, lam=1.0, n=4, N0=125, eta=0.5,
+                       precision="fp64"):
+        """field_vjp() after a field() call on the same inputs and parameters, reusing
+        its final local expansions (the tree and lists are recomputed)."""
+        pos, mom = _f64(pos), _f64(mom)
+        N = pos.shape[0]
+        gx, gp = np.empty((N, 3)), np.empty((N, 3))
+        st = Stats()
+        _check(lib().fmm4rb_field_pullback(self._h, N, pos, mom, charge, lam, n, N0, eta,
+                                           _PREC[precision], _f64(Ebar), _f64(Bbar), gx, gp,
+                                           C.byref(st)))
+        return gx, gp, st
+
+    def field_pullback_device(self, N, pos_ptr, mom_ptr, Ebar_ptr, Bbar_ptr, gpos_ptr, gmom_ptr,
+                              charge=-1.0, lam=1.0, n=4, N0=125, eta=0.5, precision="fp64"):
+        st = Stats()
+        _check(lib().fmm4rb_field_pullback_device(self._h, N, pos_ptr, mom_ptr, charge, lam, n,
+                                                  N0, eta, _PREC[precision], Ebar_ptr, Bbar_ptr,
+                                                  gpos_ptr, gmom_ptr, C.byref(st)))
         return st
 
     def bruteforce_vjp(self, pos, mom, Ebar, Bbar, charge=-1.0, lam=1.0):
