@@ -38,8 +38,10 @@ typedef struct fmm4rb_ctx fmm4rb_ctx;
 
 /* Per-call counters and stage times (HIP events).  The near field (P2P, and
  * P2P^T in reverse mode) runs on a second stream of the context, concurrently
- * with the far-field stages: ms_p2p is its own duration, ms_downward includes
- * waiting for it, and ms_total is the wall time of the whole call. */
+ * with the far-field stages, and the upward pass on a third one beside the list
+ * build: ms_p2p and ms_upward are their own durations, the other stages include
+ * the time they share the GPU, ms_downward includes waiting for P2P, and ms_total
+ * is the wall time of the whole call. */
 typedef struct {
     int64_t N, N0, ncluster, nleaf;
     int32_t n, precision, maxlevel, traversal_iterations;
@@ -56,8 +58,9 @@ const char *fmm4rb_last_error(void);
 
 /* No reference counterpart: the reference uses CUDA.jl's implicit context. */
 /* A context owns three HIP streams (default priority): its own, one for the near
- * field, and one for the reverse mode's L2P^T/L2L^T geometry, each forked from and
- * joined back into the stream the call runs on. */
+ * field, and one for the upward pass beside the list build (and the reverse mode's
+ * L2P^T/L2L^T geometry), each forked from and joined back into the stream the call
+ * runs on. */
 int fmm4rb_create(fmm4rb_ctx **ctx, int device);
 int fmm4rb_destroy(fmm4rb_ctx *ctx);
 /* Run the context's work on a caller stream (hipStream_t; NULL restores the
