@@ -533,7 +533,7 @@ struct AShape {
 // FP64 and rounded once, the terms run in FP32 (v_rsq_f32) and every source
 // cluster's FP32 sums are added into FP64 (mp holds float4 multipoles then).
 //
-// Source-node coordinates (NP <= 6): the source cluster's dl/dY_s[b] is minus the
+// Source-node coordinates (NP <= 7): the source cluster's dl/dY_s[b] is minus the
 // same per-pair term J^T u summed over the target nodes, and only its sums over
 // the nodes with index m on axis d are needed.  Per source cluster s, each lane
 // groups its terms by the source node's x index (column sums), y index (row
@@ -542,15 +542,16 @@ struct AShape {
 // (posT), so that the source side only reads them.
 template <int NP>
 struct YSplit {
-    static constexpr bool on = NP <= 6;  // 3 NP x BLK doubles of LDS <= 36 KB
+    static constexpr bool on = NP <= 7;  // 3 NP x BLK doubles of LDS: 36 KB (NP 6), 63 KB (NP 7)
 };
 
 // Occupancy: the FP64 form with the source-node sums (4 <= NP <= 6) is held to 3 waves
 // per SIMD (168 VGPRs, no spills): 167.5 -> 162.6 ms at N = 2^22, n = 4.  At NP = 7 the
-// same bound made it 15% slower (rematerialisation), so larger NP keep the default.
+// same bound made it 15% slower (rematerialisation), so larger NP keep the default (NP = 7
+// is held to 3 waves per SIMD by its 78 KB of LDS anyway).
 template <int NP, bool MIXED>
 struct AdjTWaves {
-    static constexpr int v = (!MIXED && NP >= 4 && YSplit<NP>::on) ? 3 : 1;
+    static constexpr int v = (!MIXED && NP >= 4 && NP <= 6) ? 3 : 1;
 };
 
 template <int NP, bool FWD, bool MIXED>
@@ -825,7 +826,7 @@ __attribute__((amdgpu_waves_per_eu(AdjTWaves<NP, MIXED>::v))) void k_adj_m2l_t(
 
 // Source side: per source node b, over the targets that list it: Mbar_s[b]
 // (= dl/dgamma, dl/dm) and dl/dY_s[b].  MIXED as in k_adj_m2l_t (the target
-// clusters' local cotangents are staged as FP32).  For NP <= 6 the target side
+// clusters' local cotangents are staged as FP32).  For NP <= 7 the target side
 // has already summed dl/dY_s per pair (ysp, in transposed-list order): only
 // the multipole cotangents are evaluated here, with 8 sums per node pair.
 template <int NP, bool MIXED>
@@ -1443,8 +1444,8 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     FMM_HIP(hipMemcpyAsync(c.d_costab.p, c.costab_host, sizeof c.costab_host, hipMemcpyHostToDevice,
                            st));
     transpose_lists(c, c.p2p_sorted, c.np2p, c.d_p2pT, c.d_p2pT_alt, c.d_p2pT_off, &c.p2pT_sorted);
-    // the target side's per-pair source-node sums (n <= 5) go to the transposed position
-    const bool ysplit = c.np <= 6;
+    // the target side's per-pair source-node sums (n <= 6) go to the transposed position
+    const bool ysplit = c.np <= 7;
     transpose_lists(c, c.m2l_sorted, c.nm2l, c.d_m2lT, c.d_m2lT_alt, c.d_m2lT_off, &c.m2lT_sorted,
                     ysplit ? &c.d_m2lT_pos : nullptr);
     if (ysplit) c.d_ysp.ensure(3 * (size_t)c.np * (size_t)std::max<int64_t>(c.nm2l, 1) * 8);

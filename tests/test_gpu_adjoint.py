@@ -112,7 +112,7 @@ def _same_topology(a, b):
     return all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
-@pytest.mark.parametrize("n", [3, 4, 7, 9])
+@pytest.mark.parametrize("n", [3, 4, 6, 7, 9])
 def test_fmm_vjp_matches_finite_differences(ctx, n):
     N0, eta = 64, 0.5
     pos, mom = W.bunch(4096, seed=55)
