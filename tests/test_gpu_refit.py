@@ -134,3 +134,35 @@ def test_refit_needs_a_matching_tree():
             c.field_refit(bad, mom, -1.0, LAM, 4, 125, 0.5)
     finally:
         c.close()
+
+
+def _edge_case(name):
+    if name == "ties":  # integer coordinates, many ties, signed zeros
+        rng = np.random.default_rng(3)
+        pos = rng.integers(0, 6, size=(5000, 3)).astype(float)
+        pos[::5, 0] = -0.0
+        return pos, np.zeros((5000, 3)), 30
+    if name == "earlyleaf":  # leaves above the deepest level
+        pos, mom = W.uniform_cube(251, seed=4)
+        return pos, mom, 125
+    if name == "single":
+        return np.array([[0.1, 0.2, 0.3]]), np.array([[0.0, 0.0, 1.0]]), 125
+    if name == "n0_1":
+        pos, mom = W.bunch(300, seed=6)
+        return pos, mom, 1
+    if name == "n0_1024":
+        pos, mom = W.bunch(20000, seed=7)
+        return pos, mom, 1024
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", ["ties", "earlyleaf", "single", "n0_1", "n0_1024"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_refit_edge_trees_bit_identical(ctx, name, prec):
+    """Every tree shape the build supports (leaves on several levels, one particle,
+    one particle per leaf, 1024-particle leaves, signed zeros on box faces): a refit on
+    the inputs of the full call reproduces its boxes and fields exactly."""
+    pos, mom, N0 = _edge_case(name)
+    E, B, _ = ctx.field(pos, mom, -1.0, 1.0, 4, N0, 0.5, prec)
+    E2, B2, _ = ctx.field_refit(pos, mom, -1.0, 1.0, 4, N0, 0.5, prec)
+    assert np.array_equal(E, E2) and np.array_equal(B, B2)
