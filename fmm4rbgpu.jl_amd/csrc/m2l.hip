@@ -25,10 +25,15 @@
 
 namespace fmm4rb {
 
-template <int NP>
+template <int NP, bool MIXED>
 struct M2LShape {
     static constexpr int NN = NP * NP * NP;
-    static constexpr int BLK = NN >= 1024 ? 1024 : ((NN + 63) / 64) * 64;
+    // NP = 7, mixed precision: 343 target nodes fill 6 waves to 89%; two lanes per
+    // node, each taking half of the source planes (4 and 3), fill 11 waves to 97%
+    // (156.6 -> 149.3 ms at N = 2^22).  FP64 (scalar-load source, 130 VGPRs) ran 25%
+    // slower that way (281 -> 353 ms) and keeps one lane per node.
+    static constexpr int S = (NP == 7 && MIXED) ? 2 : 1;
+    static constexpr int BLK = S * NN >= 1024 ? 1024 : ((S * NN + 63) / 64) * 64;
     static constexpr int TPT = (NN + BLK - 1) / BLK;
 };
 
@@ -47,7 +52,7 @@ __device__ inline float rsq<float>(float d) { return __builtin_amdgcn_rsqf(d); }
 // The first node of every row is peeled so that no row sum needs a reset.
 template <int NP, typename T, typename V4>
 __device__ inline void m2l_pair(double tx, double ty, double tz, const double *sn, const V4 *smp,
-                                T px, T py, T pz, double *acc) {
+                                T px, T py, T pz, double *acc, int sk0 = 0, int sk1 = NP) {
     // FP64 with n > 5: squares formed in the multiply-adds and rows not unrolled
     // (122 instead of 164 VGPRs: FP64 M2L at n = 6 took 307 ms instead of 421 ms at
     // N = 2^22); otherwise precomputed squares and unrolled rows (faster there).
@@ -64,7 +69,7 @@ __device__ inline void m2l_pair(double tx, double ty, double tz, const double *s
         }
     }
     T H = 0, Uy = 0, Uz = 0, e1 = 0, e2 = 0, b0 = 0, b1 = 0, b2 = 0;
-    for (int sk = 0; sk < NP; ++sk) {
+    for (int sk = sk0; sk < sk1; ++sk) {
         const T dz = (T)(tz - sn[2 * NP + sk]);
         const T dz2 = LEAN ? fma(dz, dz, (T)EPS64) : dz * dz, pzdz = pz * dz;
         T Gp = 0, Axp = 0, Ayp = 0;
@@ -121,20 +126,28 @@ __device__ inline void m2l_pair(double tx, double ty, double tz, const double *s
 }
 
 template <int NP, bool MIXED>
-__global__ __launch_bounds__(M2LShape<NP>::BLK) void k_m2l(
+__global__ __launch_bounds__((M2LShape<NP, MIXED>::BLK)) void k_m2l(
     const int32_t *__restrict__ off, const uint64_t *__restrict__ keys, int sb,
     const double *__restrict__ ncoord, const void *__restrict__ mpv,
     const double *__restrict__ pavg4, double *__restrict__ loc) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using V4 = typename std::conditional<MIXED, float4, double4>::type;
-    constexpr int NN = M2LShape<NP>::NN, BLK = M2LShape<NP>::BLK, TPT = M2LShape<NP>::TPT;
+    using Shp = M2LShape<NP, MIXED>;
+    constexpr int NN = Shp::NN, BLK = Shp::BLK, TPT = Shp::TPT, S = Shp::S;
+    static_assert(S == 1 || TPT == 1, "the plane split assumes one node per lane");
     // MIXED: the source staged in LDS (16 B per node read per pair); FP64: read
     // with wave-uniform addresses through the scalar cache (32 B per node per pair
     // would take most of the LDS bandwidth)
     __shared__ V4 lmp[MIXED ? NN : 1];
     __shared__ double lsn[MIXED ? 3 * NP : 1];
+    __shared__ double comb[S > 1 ? NN : 1][6];  // the second half's sums, per node
     const V4 *mp = reinterpret_cast<const V4 *>(mpv);
     const int tid = threadIdx.x;
+    // lane tid: node tid % NN, planes [sk0, sk1) of part tid / NN (S = 1: all planes)
+    const int part = S > 1 ? tid / NN : 0;
+    const int sk0 = S > 1 ? (part == 0 ? 0 : (NP + 1) / 2) : 0;
+    const int sk1 = S > 1 ? (part == 0 ? (NP + 1) / 2 : NP) : NP;
+    const int lnode = S > 1 ? tid - part * NN : tid;  // this lane's node (S > 1)
     const int32_t t = xcd_item(blockIdx.x, gridDim.x);
     const int32_t beg = off[t], end = off[t + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
@@ -142,7 +155,7 @@ __global__ __launch_bounds__(M2LShape<NP>::BLK) void k_m2l(
     double tx[TPT], ty[TPT], tz[TPT], acc[TPT][6];
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
-        const int node = min(tid + q * BLK, NN - 1);
+        const int node = S > 1 ? min(lnode, NN - 1) : min(tid + q * BLK, NN - 1);
         const int i = node % NP, j = (node / NP) % NP, k = node / (NP * NP);
         tx[q] = ncoord[((int64_t)t * 3 + 0) * NP + i];
         ty[q] = ncoord[((int64_t)t * 3 + 1) * NP + j];
@@ -162,9 +175,23 @@ __global__ __launch_bounds__(M2LShape<NP>::BLK) void k_m2l(
             sn = lsn;
             smp = lmp;
         }
+        if constexpr (S > 1) {
+            if (part < S) m2l_pair<NP, T, V4>(tx[0], ty[0], tz[0], sn, smp, px, py, pz, acc[0], sk0, sk1);
+        } else {
 #pragma unroll
-        for (int q = 0; q < TPT; ++q)
-            if (tid + q * BLK < NN) m2l_pair<NP, T, V4>(tx[q], ty[q], tz[q], sn, smp, px, py, pz, acc[q]);
+            for (int q = 0; q < TPT; ++q)
+                if (tid + q * BLK < NN) m2l_pair<NP, T, V4>(tx[q], ty[q], tz[q], sn, smp, px, py, pz, acc[q]);
+        }
+    }
+    if constexpr (S > 1) {  // second half into the first, fixed order
+        if (part == 1)
+            for (int m = 0; m < 6; ++m) comb[lnode][m] = acc[0][m];
+        __syncthreads();
+        if (part == 0) {
+            double *o = loc + ((int64_t)t * NN + lnode) * 6;
+            for (int m = 0; m < 6; ++m) o[m] = acc[0][m] + comb[lnode][m];
+        }
+        return;
     }
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
@@ -184,13 +211,13 @@ void launch_m2l(Ctx &c, const int32_t *off, const uint64_t *keys, int sb, int64_
     const unsigned grid = (unsigned)ngroups_grid;
 #define M2L_L(NP)                                                                                \
     if (mixed)                                                                                   \
-        hipLaunchKernelGGL((k_m2l<NP, true>), dim3(grid), dim3(M2LShape<NP>::BLK), 0, st, off,   \
-                           keys, sb, c.d_ncoord.as<double>(), mp, c.d_pavg.as<double>(),         \
+        hipLaunchKernelGGL((k_m2l<NP, true>), dim3(grid), dim3(M2LShape<NP, true>::BLK), 0, st, \
+                           off, keys, sb, c.d_ncoord.as<double>(), mp, c.d_pavg.as<double>(),    \
                            c.d_loc.as<double>());                                                \
     else                                                                                         \
-        hipLaunchKernelGGL((k_m2l<NP, false>), dim3(grid), dim3(M2LShape<NP>::BLK), 0, st, off,  \
-                           keys, sb, c.d_ncoord.as<double>(), mp, c.d_pavg.as<double>(),         \
-                           c.d_loc.as<double>());
+        hipLaunchKernelGGL((k_m2l<NP, false>), dim3(grid), dim3(M2LShape<NP, false>::BLK), 0,   \
+                           st, off, keys, sb, c.d_ncoord.as<double>(), mp,                       \
+                           c.d_pavg.as<double>(), c.d_loc.as<double>());
     switch (c.np) {
         case 2: M2L_L(2); break;
         case 3: M2L_L(3); break;
