@@ -31,7 +31,8 @@ struct M2LShape {
     // NP = 7, mixed precision: 343 target nodes fill 6 waves to 89%; two lanes per
     // node, each taking half of the source planes (4 and 3), fill 11 waves to 97%
     // (156.6 -> 149.3 ms at N = 2^22).  FP64 (scalar-load source, 130 VGPRs) ran 25%
-    // slower that way (281 -> 353 ms) and keeps one lane per node.
+    // slower that way (281 -> 353 ms) and keeps one lane per node; so did mixed NP = 6
+    // (216 nodes, 4 waves at 84% -> 7 at 96%: 52.3 -> 54.2 ms, fewer resident waves).
     static constexpr int S = (NP == 7 && MIXED) ? 2 : 1;
     static constexpr int BLK = S * NN >= 1024 ? 1024 : ((S * NN + 63) / 64) * 64;
     static constexpr int TPT = (NN + BLK - 1) / BLK;
