@@ -383,14 +383,19 @@ static void run_vjp(Ctx &c, int64_t N, const double *pos, const double *mom, dou
     if (saved_locals && !(c.loc_valid && c.loc_N == N && c.loc_N0 == N0 && c.loc_n == n &&
                           c.loc_prec == precision && c.loc_eta == eta))
         fail(FMM4RB_EINVAL, nofwd);
-    if (!saved_locals) c.loc_valid = false;
-    c.tree_valid = false;
+    if (!saved_locals) {
+        c.loc_valid = false;
+        c.tree_valid = false;
+    }
     prepare(c, N, N0, n, precision);
     hipStream_t st = c.stream;
     const double amp = (AMP_RE * charge) / lambda;
     FMM_HIP(hipEventRecord(c.ev[0], st));
     stage_pavg(c, pos, mom);
+    // a refused pullback leaves the context as it was (same parameters: prepare
+    // changed nothing); past this point the tree is being rebuilt
     if (saved_locals && c.in_digest != c.loc_digest) fail(FMM4RB_EINVAL, nofwd);
+    c.tree_valid = false;
     FMM_HIP(hipEventRecord(c.ev[1], st));
     build_tree_device(c, pos, mom, N, N0);
     launch_node_geo(c);

@@ -83,6 +83,9 @@ def test_pullback_needs_the_forward_call():
                                  args["N0"], args["eta"], args["precision"])
             assert e.value.code == 1
         c.field_pullback(pos, mom, Eb, Bb, -1.0, LAM, 4, 125, 0.5)  # still valid after refusals
+        with pytest.raises(F.FMMError):
+            c.field_pullback(moved, mom, Eb, Bb, -1.0, LAM, 4, 125, 0.5)
+        c.field_refit(pos, mom, -1.0, LAM, 4, 125, 0.5)  # the tree survives a refused pullback
         c.field_vjp(pos, mom, Eb, Bb, -1.0, LAM, 4, 125, 0.5)  # recomputes the locals itself
         with pytest.raises(F.FMMError) as e:
             c.field_pullback(pos, mom, Eb, Bb, -1.0, LAM, 4, 125, 0.5)
