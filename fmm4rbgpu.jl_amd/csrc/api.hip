@@ -13,6 +13,9 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <system_error>
+#include <thread>
+#include <vector>
 
 #include "fmm4rb_internal.hpp"
 
@@ -442,9 +445,104 @@ static void run_vjp(Ctx &c, int64_t N, const double *pos, const double *mom, dou
 
 using namespace fmm4rb;
 
+// Host <-> device copies of the host-pointer entry points go through two pinned
+// chunks: host threads copy one chunk between the caller's (pageable) memory and a
+// pinned buffer while the DMA engine moves the other.  Measured on MI355X for 100 MB:
+// pageable copies through the runtime 3.6 ms H2D and 6.1 ms D2H; pinned DMA 1.8 ms.
+namespace {
+constexpr size_t STAGE_CHUNK = (size_t)64 << 20;
+
+void par_memcpy(void *dst, const void *src, size_t bytes) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t T = std::max<size_t>(1, std::min<size_t>(8, hw ? hw : 1));
+    if (T == 1 || bytes < ((size_t)4 << 20)) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    // this thread copies [0, per); helpers take the following slices; whatever no
+    // helper took (a thread that could not start) is copied here too
+    const size_t per = (bytes + T - 1) / T;
+    std::vector<std::thread> th;
+    size_t next = per;
+    try {
+        while (next < bytes && th.size() + 1 < T) {
+            const size_t o = next, m = std::min(per, bytes - o);
+            th.emplace_back([=] { std::memcpy((char *)dst + o, (const char *)src + o, m); });
+            next = o + m;
+        }
+    } catch (const std::system_error &) {
+    }
+    std::memcpy(dst, src, per);
+    if (next < bytes) std::memcpy((char *)dst + next, (const char *)src + next, bytes - next);
+    for (auto &x : th) x.join();
+}
+}  // namespace
+
+struct HostStage {
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {};
+    bool used[2] = {false, false};
+    void init() {
+        if (buf[0]) return;
+        for (int b = 0; b < 2; ++b) {
+            FMM_HIP(hipHostMalloc(&buf[b], STAGE_CHUNK, hipHostMallocDefault));
+            FMM_HIP(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming));
+        }
+    }
+    void release() {
+        for (int b = 0; b < 2; ++b) {
+            if (ev[b]) (void)hipEventSynchronize(ev[b]);
+            if (buf[b]) (void)hipHostFree(buf[b]);
+            if (ev[b]) (void)hipEventDestroy(ev[b]);
+            buf[b] = nullptr;
+            ev[b] = nullptr;
+            used[b] = false;
+        }
+    }
+    // ordered on st; returns once the caller's memory may be reused
+    void h2d(void *d, const void *h, size_t bytes, hipStream_t st) {
+        init();
+        for (size_t off = 0, k = 0; off < bytes; off += STAGE_CHUNK, ++k) {
+            const size_t n = std::min(STAGE_CHUNK, bytes - off);
+            const int b = (int)(k & 1);
+            if (used[b]) FMM_HIP(hipEventSynchronize(ev[b]));  // its last DMA has read it
+            par_memcpy(buf[b], (const char *)h + off, n);
+            FMM_HIP(hipMemcpyAsync((char *)d + off, buf[b], n, hipMemcpyHostToDevice, st));
+            FMM_HIP(hipEventRecord(ev[b], st));
+            used[b] = true;
+        }
+    }
+    // chunk k is moved by DMA while chunk k-1 is copied out to the caller
+    void d2h(void *h, const void *d, size_t bytes, hipStream_t st) {
+        init();
+        size_t poff = 0, pn = 0;
+        int pb = -1;
+        for (size_t off = 0, k = 0; off < bytes; off += STAGE_CHUNK, ++k) {
+            const size_t n = std::min(STAGE_CHUNK, bytes - off);
+            const int b = (int)(k & 1);
+            if (used[b]) FMM_HIP(hipEventSynchronize(ev[b]));  // e.g. an H2D on another stream
+            FMM_HIP(hipMemcpyAsync(buf[b], (const char *)d + off, n, hipMemcpyDeviceToHost, st));
+            FMM_HIP(hipEventRecord(ev[b], st));
+            used[b] = true;
+            if (pb >= 0) {
+                FMM_HIP(hipEventSynchronize(ev[pb]));
+                par_memcpy((char *)h + poff, buf[pb], pn);
+            }
+            pb = b;
+            poff = off;
+            pn = n;
+        }
+        if (pb >= 0) {
+            FMM_HIP(hipEventSynchronize(ev[pb]));
+            par_memcpy((char *)h + poff, buf[pb], pn);
+        }
+    }
+};
+
 struct fmm4rb_ctx {
     Ctx c;
     DevBuf h_pos, h_mom, h_E, h_B, h_tg, h_gpos, h_gmom, h_ob;
+    HostStage stage;
 };
 
 #define FMM_API_BEGIN try {
@@ -517,9 +615,10 @@ int fmm4rb_destroy(fmm4rb_ctx *ctx) {
                       &c.d_p2pT_off, &c.d_m2lT_off,
                       &c.d_front0, &c.d_front1, &c.d_tcnt, &c.d_p2p, &c.d_p2p_alt, &c.d_m2l,
                       &c.d_m2l_alt, &c.d_p2p_off, &c.d_m2l_off, &c.d_mp4, &c.d_mp4f, &c.d_loc,
-                      &c.d_near, &ctx->h_pos, &ctx->h_mom, &ctx->h_E, &ctx->h_B, &ctx->h_tg,
-                      &ctx->h_gpos, &ctx->h_gmom, &ctx->h_ob};
+                      &c.d_near, &c.d_m2lT_pos, &c.d_ysp, &ctx->h_pos, &ctx->h_mom, &ctx->h_E,
+                      &ctx->h_B, &ctx->h_tg, &ctx->h_gpos, &ctx->h_gmom, &ctx->h_ob};
     for (DevBuf *b : bufs) b->release();
+    ctx->stage.release();
     for (auto &e : c.ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : c.evx)
@@ -578,13 +677,12 @@ static void field_host(fmm4rb_ctx *ctx, int64_t N, const double *pos, const doub
     ctx->h_mom.ensure(bytes);
     ctx->h_E.ensure(bytes);
     ctx->h_B.ensure(bytes);
-    FMM_HIP(hipMemcpyAsync(ctx->h_pos.p, pos, bytes, hipMemcpyHostToDevice, st));
-    FMM_HIP(hipMemcpyAsync(ctx->h_mom.p, mom, bytes, hipMemcpyHostToDevice, st));
+    ctx->stage.h2d(ctx->h_pos.p, pos, bytes, st);
+    ctx->stage.h2d(ctx->h_mom.p, mom, bytes, st);
     run_field(ctx->c, N, ctx->h_pos.as<double>(), ctx->h_mom.as<double>(), charge, lambda, n, N0,
               eta, precision, ctx->h_E.as<double>(), ctx->h_B.as<double>(), stats, refit);
-    FMM_HIP(hipMemcpyAsync(E, ctx->h_E.p, bytes, hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(B, ctx->h_B.p, bytes, hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipStreamSynchronize(st));
+    ctx->stage.d2h(E, ctx->h_E.p, bytes, st);
+    ctx->stage.d2h(B, ctx->h_B.p, bytes, st);
 }
 
 int fmm4rb_field(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom, double charge,
@@ -653,16 +751,15 @@ static void vjp_host(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double
     const size_t bytes = 3 * (size_t)N * sizeof(double);
     DevBuf *b[] = {&ctx->h_pos, &ctx->h_mom, &ctx->h_E, &ctx->h_B, &ctx->h_gpos, &ctx->h_gmom};
     for (DevBuf *x : b) x->ensure(bytes);
-    FMM_HIP(hipMemcpyAsync(ctx->h_pos.p, pos, bytes, hipMemcpyHostToDevice, st));
-    FMM_HIP(hipMemcpyAsync(ctx->h_mom.p, mom, bytes, hipMemcpyHostToDevice, st));
-    FMM_HIP(hipMemcpyAsync(ctx->h_E.p, Ebar, bytes, hipMemcpyHostToDevice, st));
-    FMM_HIP(hipMemcpyAsync(ctx->h_B.p, Bbar, bytes, hipMemcpyHostToDevice, st));
+    ctx->stage.h2d(ctx->h_pos.p, pos, bytes, st);
+    ctx->stage.h2d(ctx->h_mom.p, mom, bytes, st);
+    ctx->stage.h2d(ctx->h_E.p, Ebar, bytes, st);
+    ctx->stage.h2d(ctx->h_B.p, Bbar, bytes, st);
     run_vjp(ctx->c, N, ctx->h_pos.as<double>(), ctx->h_mom.as<double>(), charge, lambda, n, N0,
             eta, precision, ctx->h_E.as<double>(), ctx->h_B.as<double>(),
             ctx->h_gpos.as<double>(), ctx->h_gmom.as<double>(), stats, saved_locals);
-    FMM_HIP(hipMemcpyAsync(gpos, ctx->h_gpos.p, bytes, hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(gmom, ctx->h_gmom.p, bytes, hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipStreamSynchronize(st));
+    ctx->stage.d2h(gpos, ctx->h_gpos.p, bytes, st);
+    ctx->stage.d2h(gmom, ctx->h_gmom.p, bytes, st);
 }
 
 int fmm4rb_field_pullback(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
