@@ -1567,6 +1567,9 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
         ADJ_NP_SWITCH(c.np, ADJ_L2LG);
         FMM_LAUNCH_CHECK();
     }
+    // the mean-momentum partials of the M2L^T target side, summed (one workgroup)
+    hipLaunchKernelGGL(k_adj_pbar, dim3(1), dim3(256), 0, sg, pbar, nc, pbar + 3 * nc);
+    FMM_LAUNCH_CHECK();
     FMM_HIP(hipEventRecord(c.evy[1], sg));
     // 4. M2L^T source side
 #define ADJ_M2LS(NP)                                                                              \
@@ -1579,8 +1582,6 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
                            0, st, c.d_m2lT_off.as<int32_t>(), c.m2lT_sorted, c.sb, ncoord, mpv,    \
                            lbar, c.d_pavg.as<double>(), mbar, xbar2, ysp)
     ADJ_NP_SWITCH(c.np, ADJ_M2LS);
-    FMM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_adj_pbar, dim3(1), dim3(256), 0, st, pbar, nc, pbar + 3 * nc);
     FMM_LAUNCH_CHECK();
     FMM_HIP(hipEventRecord(c.ev[9], st));
     // 5. M2M^T values top-down (parents on level l-1 for l = 1 .. ml), then the
