@@ -106,13 +106,13 @@ __device__ inline double axis_adj(const double *w, const double *dw, const doubl
 
 constexpr int AT = 128;  // threads of the leaf / transfer adjoint kernels
 
-// Sum per-thread partials red[AT][W] into out[W] (+=), fixed order.
-template <int W>
+// Sum per-thread partials red[T][W] of a T-thread block into out[W] (+=), fixed order.
+template <int W, int T = AT>
 __device__ inline void block_sum_into(double (*red)[W], double *out) {
     __syncthreads();
-    for (int j = threadIdx.x; j < W; j += AT) {
+    for (int j = threadIdx.x; j < W; j += T) {
         double s = 0.0;
-        for (int t = 0; t < AT; ++t) s += red[t][j];
+        for (int t = 0; t < T; ++t) s += red[t][j];
         out[j] += s;
     }
     __syncthreads();
@@ -124,8 +124,10 @@ __device__ inline void block_sum_into(double (*red)[W], double *out) {
 // MODE 1 (P2M^T): A = Mbar (4 comps), o_i = (g_i, p_i);
 //   qbar_i = sum_a A[a] W_i[a] -> gp_i += qbar_p + qbar_g p_i/g_i;  gx_i, Xbar as above.
 // PART bit 1: the gradient (geometry) terms; bit 2: the Lbar values (MODE 0).
-template <int NP, int MODE, int PART>
-__global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ leaves,
+// LT threads: the geometry takes one thread per particle, so leaves of at most 64
+// particles are launched with one wave (LT = 64) and larger ones with two.
+template <int NP, int MODE, int PART, int LT = AT>
+__global__ __launch_bounds__(LT) void k_adj_leaf(const int32_t *__restrict__ leaves,
                                                  const int32_t *__restrict__ lo,
                                                  const int32_t *__restrict__ cnt,
                                                  const double *__restrict__ sp8,
@@ -135,12 +137,12 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
                                                  double *__restrict__ Abar, double *__restrict__ gx,
                                                  double *__restrict__ gp,
                                                  double *__restrict__ Xbar) {
-    constexpr int NN = NP * NP * NP, NC = MODE == 0 ? 6 : 4, TPT = (NN + AT - 1) / AT;
+    constexpr int NN = NP * NP * NP, NC = MODE == 0 ? 6 : 4, TPT = (NN + LT - 1) / LT;
     // the leaf's NN x NC coefficients are read by every particle: staged in LDS (n <= 5)
     constexpr bool STAGE = NN * NC * 8 <= 8192;
     __shared__ double nd[3 * NP];
-    __shared__ double red[AT][3 * NP];  // Xbar partials, then the weight tiles
-    __shared__ double Ot[AT][6];
+    __shared__ double red[LT][3 * NP];  // Xbar partials, then the weight tiles
+    __shared__ double Ot[LT][6];
     __shared__ double sA[STAGE ? NN * NC : 1];
     double(*Wt)[3 * NP] = red;
     const int tid = threadIdx.x;
@@ -152,19 +154,22 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
     for (int j = 0; j < 3 * NP; ++j) xbl[j] = 0.0;
     const double *Ak = A + (int64_t)k * NN * NC;
     if (STAGE && (PART & 1)) {
-        for (int j = tid; j < NN * NC; j += AT) sA[j] = Ak[j];
+        for (int j = tid; j < NN * NC; j += LT) sA[j] = Ak[j];
         Ak = sA;
     }
     __syncthreads();
     // per-particle gradient terms
-    for (int p = tid; (PART & 1) && p < c; p += AT) {
+    for (int p = tid; (PART & 1) && p < c; p += LT) {
         const int64_t i = a + p;
         const double *x = sp8 + 8 * i;
-        double w[3][NP], dw[3][NP];
+        // the weights only: their derivatives and 1/(x - X) are not kept across the
+        // node loop (registers, and so resident waves) but formed again below, with
+        // the same operations
+        double w[3][NP];
         int fl[3];
         for (int d = 0; d < 3; ++d) {
-            double ri[NP];  // not kept across the node loop (registers): recomputed below
-            fl[d] = lgw_d<NP>(x[d], nd + d * NP, w[d], dw[d], ri);
+            double dw[NP], ri[NP];
+            fl[d] = lgw_d<NP>(x[d], nd + d * NP, w[d], dw, ri);
         }
         double o[NC];
         if (MODE == 0) {
@@ -219,11 +224,9 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
         }
         double g[3];
         for (int d = 0; d < 3; ++d) {
-            double ri[NP];
-            if (fl[d] < 0)
-#pragma unroll
-                for (int m = 0; m < NP; ++m) ri[m] = 1.0 / (x[d] - nd[d * NP + m]);
-            g[d] = axis_adj<NP>(w[d], dw[d], ri, fl[d], T[d], xbl + d * NP);
+            double wd[NP], dw[NP], ri[NP];
+            lgw_d<NP>(x[d], nd + d * NP, wd, dw, ri);
+            g[d] = axis_adj<NP>(w[d], dw, ri, fl[d], T[d], xbl + d * NP);
         }
         for (int d = 0; d < 3; ++d) gx[3 * i + d] += g[d];
         if (MODE == 1) {
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
     if (PART & 1) {
 #pragma unroll
         for (int j = 0; j < 3 * NP; ++j) red[tid][j] = xbl[j];
-        block_sum_into<3 * NP>(red, Xbar + (int64_t)k * 3 * NP);
+        block_sum_into<3 * NP, LT>(red, Xbar + (int64_t)k * 3 * NP);
     }
     if (MODE == 0 && (PART & 2)) {  // Lbar[leaf][a] = sum_i o_i W_i[a]
         double acc[TPT][6];
@@ -242,7 +245,7 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
         for (int q = 0; q < TPT; ++q)
 #pragma unroll
             for (int m = 0; m < 6; ++m) acc[q][m] = 0.0;
-        for (int p0 = 0; p0 < c; p0 += AT) {
+        for (int p0 = 0; p0 < c; p0 += LT) {
             __syncthreads();
             const int p = p0 + tid;
             if (p < c) {
@@ -253,10 +256,10 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
                 for (int m = 0; m < 6; ++m) Ot[tid][m] = ob[6 * i + m];
             }
             __syncthreads();
-            const int mm = min(AT, c - p0);
+            const int mm = min(LT, c - p0);
 #pragma unroll
             for (int q = 0; q < TPT; ++q) {
-                const int node = tid + q * AT;
+                const int node = tid + q * LT;
                 if (node < NN) {
                     const int ix = node % NP, jy = (node / NP) % NP, kz = node / (NP * NP);
                     for (int pp = 0; pp < mm; ++pp) {
@@ -269,11 +272,31 @@ __global__ __launch_bounds__(AT) void k_adj_leaf(const int32_t *__restrict__ lea
         }
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
-            const int node = tid + q * AT;
+            const int node = tid + q * LT;
             if (node < NN)
 #pragma unroll
                 for (int m = 0; m < 6; ++m) Abar[((int64_t)k * NN + node) * 6 + m] = acc[q][m];
         }
+    }
+}
+
+// One contraction pass of a transfer kernel: dst[cc][n] = term(cc, n) for the E
+// elements (cc, n) of the block, AT threads.  Every element of the thread is
+// evaluated before any is stored: the source and destination tiles share one LDS
+// array, so interleaved stores would hold back the next element's loads.
+template <int NN, int E, class F>
+__device__ inline void lds_pass(double (*dst)[NN], F term) {
+    constexpr int EPT = (E + AT - 1) / AT;
+    double sv[EPT];
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+        const int e = (int)threadIdx.x + r * AT;
+        sv[r] = e < E ? term(e / NN, e % NN) : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+        const int e = (int)threadIdx.x + r * AT;
+        if (e < E) dst[e / NN][e % NN] = sv[r];
     }
 }
 
@@ -317,9 +340,8 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
     const int32_t P = parents[blockIdx.x];
     if (parity >= 0 && ((31 - __clz(P + 1)) & 1) != parity) return;  // heap level of P
     if (tid < 3 * NP) pn[tid] = ncoord[(int64_t)P * 3 * NP + tid];
-    double xbP[3 * NP];  // this thread's share of dl/dX_P
-#pragma unroll
-    for (int j = 0; j < 3 * NP; ++j) xbP[j] = 0.0;
+    // dl/dX_P terms per child: xPs[ch][d NP + a][m] from child node a on axis d
+    __shared__ double xPs[2][3 * NP][NP];
     double accP[TPT][NC];  // MODE 0: Lbar_P
 #pragma unroll
     for (int q = 0; q < TPT; ++q)
@@ -335,10 +357,13 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
             double ri[NP];
             fl[d][ai] = lgw_d<NP>(cn[tid], pn + d * NP, &Om[d][ai][0], &dOm[d][ai][0], ri);
         }
-        // dl/dOm accumulated by the thread owning the (a_d, b_d) pair
+        // dl/dOm of the (a_d, b_d) pair go, summed by GO lanes (gl = 0 .. GO-1) that
+        // take every GO-th term and meet in xor shuffles; lane gl = 0 keeps it
+        constexpr int GO = NP * NP * 4 <= AT ? 4 : (NP * NP * 2 <= AT ? 2 : 1);
         double gO[3] = {0.0, 0.0, 0.0};
-        const int pa = tid / NP, pb = tid % NP;
-        const bool own = tid < NP * NP;
+        const int go = tid / GO, gl = tid % GO;
+        const int pa = go / NP, pb = go % NP;
+        const bool own = go < NP * NP;
         double accC[TPT][NC];  // MODE 1: Mbar_c increment
 #pragma unroll
         for (int q = 0; q < TPT; ++q)
@@ -355,21 +380,22 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
             // ---- value propagation
             if (!(PART & 2)) {
             } else if (MODE == 0) {  // Lbar_P[b] += sum_a Om(a,b) Lbar_c[a]: contract a_x, a_y, a_z
-                for (int e = tid; e < CG * NN; e += AT) {  // t1[bx, ay, az]
-                    const int cc = e / NN, n = e % NN, bx = n % NP, r = n / NP;
+                lds_pass<NN, CG * NN>(t1, [&](int cc, int n) {  // t1[bx, ay, az]
+                    const int bx = n % NP, r = n / NP;
                     double s = 0.0;
+#pragma unroll
                     for (int ax = 0; ax < NP; ++ax) s += Om[0][ax][bx] * bU[cc][ax + NP * r];
-                    t1[cc][n] = s;
-                }
+                    return s;
+                });
                 __syncthreads();
-                for (int e = tid; e < CG * NN; e += AT) {  // t2[bx, by, az]
-                    const int cc = e / NN, n = e % NN;
+                lds_pass<NN, CG * NN>(t2, [&](int cc, int n) {  // t2[bx, by, az]
                     const int bx = n % NP, by = (n / NP) % NP, az = n / (NP * NP);
                     double s = 0.0;
+#pragma unroll
                     for (int ay = 0; ay < NP; ++ay)
                         s += Om[1][ay][by] * t1[cc][bx + NP * (ay + NP * az)];
-                    t2[cc][n] = s;
-                }
+                    return s;
+                });
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < TPT; ++q) {
@@ -386,21 +412,22 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
                     }
                 }
             } else {  // Mbar_c[a] += sum_b Om(a,b) Mbar_P[b]: contract b_x, b_y, b_z
-                for (int e = tid; e < CG * NN; e += AT) {  // t1[ax, by, bz]
-                    const int cc = e / NN, n = e % NN, ax = n % NP, r = n / NP;
+                lds_pass<NN, CG * NN>(t1, [&](int cc, int n) {  // t1[ax, by, bz]
+                    const int ax = n % NP, r = n / NP;
                     double s = 0.0;
+#pragma unroll
                     for (int bx = 0; bx < NP; ++bx) s += Om[0][ax][bx] * bV[cc][bx + NP * r];
-                    t1[cc][n] = s;
-                }
+                    return s;
+                });
                 __syncthreads();
-                for (int e = tid; e < CG * NN; e += AT) {  // t2[ax, ay, bz]
-                    const int cc = e / NN, n = e % NN;
+                lds_pass<NN, CG * NN>(t2, [&](int cc, int n) {  // t2[ax, ay, bz]
                     const int ax = n % NP, ay = (n / NP) % NP, bz = n / (NP * NP);
                     double s = 0.0;
+#pragma unroll
                     for (int by = 0; by < NP; ++by)
                         s += Om[1][ay][by] * t1[cc][ax + NP * (by + NP * bz)];
-                    t2[cc][n] = s;
-                }
+                    return s;
+                });
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < TPT; ++q) {
@@ -418,50 +445,52 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
                 }
             }
             // ---- geometry: three axes
-            for (int ax3 = 0; (PART & 1) && ax3 < 3; ++ax3) {
+            // (unrolled: the axes are constants, and node indices move by fixed strides)
+#pragma unroll
+            for (int ax3 = 0; ax3 < 3; ++ax3) {
+                if (!(PART & 1)) break;
                 // V'[..] = V contracted over the two other axes with Om (b -> a)
                 const int d1 = (ax3 + 1) % 3, d2 = (ax3 + 2) % 3;
+                const int st1 = d1 == 0 ? 1 : (d1 == 1 ? NP : NP * NP);
+                const int st2 = d2 == 0 ? 1 : (d2 == 1 ? NP : NP * NP);
                 __syncthreads();
-                for (int e = tid; e < CG * NN; e += AT) {  // contract axis d1: index b_d1 -> a_d1
-                    const int cc = e / NN, n = e % NN;
-                    int id[3] = {n % NP, (n / NP) % NP, n / (NP * NP)};
-                    const int a1 = id[d1];
+                lds_pass<NN, CG * NN>(t1, [&](int cc, int n) {  // contract axis d1: b_d1 -> a_d1
+                    const int a1 = (n / st1) % NP, n0 = n - a1 * st1;
                     double s = 0.0;
-                    for (int b1 = 0; b1 < NP; ++b1) {
-                        id[d1] = b1;
-                        s += Om[d1][a1][b1] * bV[cc][id[0] + NP * (id[1] + NP * id[2])];
-                    }
-                    t1[cc][n] = s;
-                }
-                __syncthreads();
-                for (int e = tid; e < CG * NN; e += AT) {  // contract axis d2
-                    const int cc = e / NN, n = e % NN;
-                    int id[3] = {n % NP, (n / NP) % NP, n / (NP * NP)};
-                    const int a2 = id[d2];
-                    double s = 0.0;
-                    for (int b2 = 0; b2 < NP; ++b2) {
-                        id[d2] = b2;
-                        s += Om[d2][a2][b2] * t1[cc][id[0] + NP * (id[1] + NP * id[2])];
-                    }
-                    t2[cc][n] = s;
-                }
-                __syncthreads();
-                if (own) {  // gO[ax3] += sum over the other a indices of U[a] t2[(a with a_ax3 -> b)]
-                    double s = 0.0;
-                    for (int u1 = 0; u1 < NP; ++u1)
-                        for (int u2 = 0; u2 < NP; ++u2) {
-                            int ia[3], ib[3];
-                            ia[ax3] = pa;
-                            ib[ax3] = pb;
-                            ia[d1] = ib[d1] = u1;
-                            ia[d2] = ib[d2] = u2;
-                            const int na = ia[0] + NP * (ia[1] + NP * ia[2]);
-                            const int nb = ib[0] + NP * (ib[1] + NP * ib[2]);
 #pragma unroll
-                            for (int cc = 0; cc < CG; ++cc) s += bU[cc][na] * t2[cc][nb];
-                        }
-                    gO[ax3] += s;
+                    for (int b1 = 0; b1 < NP; ++b1) s += Om[d1][a1][b1] * bV[cc][n0 + b1 * st1];
+                    return s;
+                });
+                __syncthreads();
+                lds_pass<NN, CG * NN>(t2, [&](int cc, int n) {  // contract axis d2
+                    const int a2 = (n / st2) % NP, n0 = n - a2 * st2;
+                    double s = 0.0;
+#pragma unroll
+                    for (int b2 = 0; b2 < NP; ++b2) s += Om[d2][a2][b2] * t1[cc][n0 + b2 * st2];
+                    return s;
+                });
+                __syncthreads();
+                // gO[ax3] += sum over the other a indices of U[a] t2[(a with a_ax3 -> b)]
+                double s = 0.0;
+#pragma unroll
+                for (int r = 0; r < (NP * NP + GO - 1) / GO; ++r) {
+                    const int u = gl + r * GO;
+                    if (own && u < NP * NP) {
+                        const int u1 = u / NP, u2 = u - u1 * NP;
+                        int ia[3], ib[3];
+                        ia[ax3] = pa;
+                        ib[ax3] = pb;
+                        ia[d1] = ib[d1] = u1;
+                        ia[d2] = ib[d2] = u2;
+                        const int na = ia[0] + NP * (ia[1] + NP * ia[2]);
+                        const int nb = ib[0] + NP * (ib[1] + NP * ib[2]);
+#pragma unroll
+                        for (int cc = 0; cc < CG; ++cc) s += bU[cc][na] * t2[cc][nb];
+                    }
                 }
+#pragma unroll
+                for (int o = 1; o < GO; o <<= 1) s += __shfl_xor(s, o);
+                gO[ax3] += s;
             }
         }
         // child's value output (MODE 1)
@@ -476,8 +505,8 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
         }
         // dl/dOm -> dl/dX_c (evaluation points) and dl/dX_P (nodes)
         __syncthreads();
-        if ((PART & 1) && own)
-            for (int d = 0; d < 3; ++d) red[tid][d] = gO[d];  // red[(a,b)][axis]
+        if ((PART & 1) && own && gl == 0)
+            for (int d = 0; d < 3; ++d) red[go][d] = gO[d];  // red[(a,b)][axis]
         __syncthreads();
         if ((PART & 1) && tid < 3 * NP) {
             const int d = tid / NP, ai = tid % NP;
@@ -490,7 +519,7 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
                 for (int m = 0; m < NP; ++m) ri[m] = 1.0 / (cn[tid] - pn[d * NP + m]);
             const double gxc = axis_adj<NP>(&Om[d][ai][0], &dOm[d][ai][0], ri, fl[d][ai], cvec, xP);
             Xbar[(int64_t)C * 3 * NP + tid] += gxc;
-            for (int m = 0; m < NP; ++m) xbP[d * NP + m] += xP[m];
+            for (int m = 0; m < NP; ++m) xPs[ch][tid][m] = xP[m];
         }
     }
     if (MODE == 0 && (PART & 2)) {
@@ -502,10 +531,14 @@ __global__ __launch_bounds__(AT) void k_adj_transfer(const int32_t *__restrict__
                 for (int m = 0; m < NC; ++m) out[((int64_t)P * NN + e) * NC + m] = accP[q][m];
         }
     }
-    if (PART & 1) {
-#pragma unroll
-        for (int j = 0; j < 3 * NP; ++j) red[tid][j] = xbP[j];
-        block_sum_into<3 * NP>(red, Xbar + (int64_t)P * 3 * NP);
+    if (PART & 1) {  // dl/dX_P[d][m]: over the child nodes a, both children per node
+        __syncthreads();
+        if (tid < 3 * NP) {
+            const int d = tid / NP, m = tid % NP;
+            double sum = 0.0;
+            for (int a = 0; a < NP; ++a) sum += xPs[0][d * NP + a][m] + xPs[1][d * NP + a][m];
+            Xbar[(int64_t)P * 3 * NP + tid] += sum;
+        }
     }
 }
 
@@ -1553,9 +1586,15 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     hipStream_t sg = c.third_stream();
     FMM_HIP(hipEventRecord(c.evy[0], st));
     FMM_HIP(hipStreamWaitEvent(sg, c.evy[0], 0));
-#define ADJ_L2PG(NP)                                                                                \
-    hipLaunchKernelGGL((k_adj_leaf<NP, 0, 1>), dim3(nleaf), dim3(AT), 0, sg, leaves, lo, cnt, sp8,  \
-                       ncoord, loc, ob, lbar, gx, gp, xbar)
+#define ADJ_L2PG_T(NP, LT)                                                                         \
+    hipLaunchKernelGGL((k_adj_leaf<NP, 0, 1, LT>), dim3(nleaf), dim3(LT), 0, sg, leaves, lo, cnt,  \
+                       sp8, ncoord, loc, ob, lbar, gx, gp, xbar)
+#define ADJ_L2PG(NP)                    \
+    if (S.maxleaf <= 64) {              \
+        ADJ_L2PG_T(NP, 64);             \
+    } else {                            \
+        ADJ_L2PG_T(NP, AT);             \
+    }
     ADJ_NP_SWITCH(c.np, ADJ_L2PG);
     FMM_LAUNCH_CHECK();
     const int64_t npar = S.split_off[S.ml];  // split nodes of levels 0 .. ml-1
@@ -1585,7 +1624,11 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     FMM_LAUNCH_CHECK();
     FMM_HIP(hipEventRecord(c.ev[9], st));
     // 5. M2M^T values top-down (parents on level l-1 for l = 1 .. ml), then the
-    //    M2M^T geometry of all levels (needs the final Mbar) in two launches
+    //    M2M^T geometry of all levels (needs the final Mbar) in two launches.  The
+    //    geometry here and P2M^T (6.) add their node-coordinate terms into xbar2
+    //    after the M2L^T source side and their particle terms into gx2/gp2 after
+    //    P2P^T, so this stream does not wait for the third one until the
+    //    bounding-box step; every buffer keeps one fixed order of additions.
     for (int l = 1; l <= S.ml; ++l) {
         const int64_t b = S.split_off[l - 1], e = S.split_off[l];
         if (e == b) continue;
@@ -1596,26 +1639,32 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
         ADJ_NP_SWITCH(c.np, ADJ_M2M);
         FMM_LAUNCH_CHECK();
     }
-    FMM_HIP(hipStreamWaitEvent(st, c.evy[1], 0));  // join: the geometry below also adds to xbar
     for (int parity = 0; parity < 2 && npar > 0; ++parity) {
 #define ADJ_M2MG(NP)                                                                               \
     hipLaunchKernelGGL((k_adj_transfer<NP, 1, 1>), dim3((unsigned)npar), dim3(AT), 0, st, allpar, \
-                       ncoord, c.d_mp4.as<double>(), mbar, mbar, xbar, parity)
+                       ncoord, c.d_mp4.as<double>(), mbar, mbar, xbar2, parity)
         ADJ_NP_SWITCH(c.np, ADJ_M2MG);
         FMM_LAUNCH_CHECK();
     }
     // 6. P2M^T
-#define ADJ_P2M(NP)                                                                                 \
-    hipLaunchKernelGGL((k_adj_leaf<NP, 1, 3>), dim3(nleaf), dim3(AT), 0, st, leaves, lo, cnt, sp8,  \
-                       ncoord, mbar, ob, (double *)nullptr, gx, gp, xbar)
+    FMM_HIP(hipStreamWaitEvent(st, c.evx[2], 0));  // join the P2P^T stream (gx2, gp2)
+#define ADJ_P2M_T(NP, LT)                                                                          \
+    hipLaunchKernelGGL((k_adj_leaf<NP, 1, 3, LT>), dim3(nleaf), dim3(LT), 0, st, leaves, lo, cnt,  \
+                       sp8, ncoord, mbar, ob, (double *)nullptr, gx2, gp2, xbar2)
+#define ADJ_P2M(NP)                     \
+    if (S.maxleaf <= 64) {              \
+        ADJ_P2M_T(NP, 64);              \
+    } else {                            \
+        ADJ_P2M_T(NP, AT);              \
+    }
     ADJ_NP_SWITCH(c.np, ADJ_P2M);
     FMM_LAUNCH_CHECK();
     // bounding boxes -> extreme particles, mean momentum, output order
+    FMM_HIP(hipStreamWaitEvent(st, c.evy[1], 0));  // join the third stream (xbar, gx, gp)
     hipLaunchKernelGGL(k_adj_bbox, dim3(nbk(nc, 128)), dim3(128), 0, st, nc, c.np,
                        c.d_flags.as<uint8_t>(), xbar, xbar2, c.d_costab.as<double>(),
                        c.d_abar.as<double>());
     FMM_LAUNCH_CHECK();
-    FMM_HIP(hipStreamWaitEvent(st, c.evx[2], 0));  // join the P2P^T stream
     hipLaunchKernelGGL(k_adj_finish, dim3(nbk(N, 256)), dim3(256), 0, st, N,
                        c.d_parind.as<int32_t>(), c.d_leafof.as<int32_t>(), c.d_ext.as<int32_t>(),
                        c.d_abar.as<double>(), pbar + 3 * nc, gx, gp, gx2, gp2, gpos, gmom);
