@@ -114,16 +114,30 @@ def _same_topology(a, b):
 
 @pytest.mark.parametrize("n", [3, 4, 6, 7, 9])
 def test_fmm_vjp_matches_finite_differences(ctx, n):
-    N0, eta = 64, 0.5
-    pos, mom = W.bunch(4096, seed=55)
-    w = np.random.default_rng(56).random(len(pos))
+    _fmm_fd_check(ctx, n, 4096, 64, 55)
+
+
+def test_fmm_vjp_matches_finite_differences_two_wave_leaves(ctx):
+    """Leaves of 93-94 particles: the leaf geometry kernels (L2P^T, P2M^T) run two
+    waves per leaf instead of one."""
+    pos, _ = W.bunch(6000, seed=58)
+    t = ctx.tree(pos, np.zeros_like(pos), 125)
+    leaf = (t["lo"] >= 0) & (t["children"][:, 0] < 0)
+    assert (t["hi"][leaf] - t["lo"][leaf] + 1).max() > 64
+    _fmm_fd_check(ctx, 4, 6000, 125, 58)
+
+
+def _fmm_fd_check(ctx, n, N, N0, seed):
+    eta = 0.5
+    pos, mom = W.bunch(N, seed=seed)
+    w = np.random.default_rng(seed + 1).random(len(pos))
     E, B, _ = ctx.field(pos, mom, -1.0, LAM, n, N0, eta)
     gx, gp, st = ctx.field_vjp(pos, mom, 2 * w[:, None] * E, 2 * w[:, None] * B, -1.0, LAM, n,
                                N0, eta)
     fields = lambda x, p: ctx.field(x, p, -1.0, LAM, n, N0, eta)[:2]
     hs = _steps(pos, mom)
     base = _topology(ctx, pos, mom, N0, eta)
-    rng = np.random.default_rng(57)
+    rng = np.random.default_rng(seed + 2)
     picks = _pick(rng, len(pos), 40)
     # bounding-box extremes: particles whose coordinate is a leaf's min or max
     t = ctx.tree(pos, mom, N0)
