@@ -1018,6 +1018,7 @@ int fmm4rb_op_m2l(fmm4rb_ctx *ctx, int n, int precision, int64_t nc, const doubl
         for (int d = 0; d < 3; ++d) mp[4 * e + 1 + d] = mmom[3 * e + d];
     }
     FMM_HIP(hipMemcpyAsync(c.d_pavg.p, pa, 32, hipMemcpyHostToDevice, st));
+    for (int d = 0; d < 4; ++d) c.pavg_host[d] = pa[d];  // launch_m2l reads it
     FMM_HIP(hipMemcpyAsync(c.d_bbox.p, bboxes, 48 * (size_t)nc, hipMemcpyHostToDevice, st));
     FMM_HIP(hipMemcpyAsync(c.d_mp4.p, mp.data(), 8 * mp.size(), hipMemcpyHostToDevice, st));
     launch_node_geo(c);

@@ -19,7 +19,12 @@
 // FP64 and rounded once to FP32, the kernel and row sums run in FP32
 // (v_rsq_f32), and each source pair's FP32 sum is added into FP64
 // accumulators.  (The reference stages the source in shared memory,
-// gpu_interaction.jl:15-38.)
+// gpu_interaction.jl:15-38.)  When |px| <= 0.1 (a beam moving mostly across x,
+// the inner loop's axis) the mixed kernel forms d = R.R + (p.R)^2 + eps as a
+// quadratic in dx, d = dx ((1 + px^2) dx + 2 px p.R_yz) + (R_yz.R_yz + (p.R_yz)^2 + eps),
+// two multiply-adds per node pair instead of three; its cross term is at most 0.1 of
+// the other two (2 |px p.R_yz dx| <= |px| (dx^2 + (p.R_yz)^2)), so the rounding
+// stays within 1.2 FP32 ulps of d.
 
 #include "fmm4rb_internal.hpp"
 
@@ -51,9 +56,11 @@ __device__ inline float rsq<float>(float d) { return __builtin_amdgcn_rsqf(d); }
 // fixed): G, A_x, A_y; per row (dy fixed): G, A_x, A_z.  So E = (H, sum_rows dy G,
 // sum_planes dz G), B = (sum dz A_y - sum dy A_z, U_z - sum dz A_x, sum dy A_x - U_y).
 // The first node of every row is peeled so that no row sum needs a reset.
-template <int NP, typename T, typename V4>
+template <int NP, typename T, typename V4, bool QUAD = false>
 __device__ inline void m2l_pair(double tx, double ty, double tz, const double *sn, const V4 *smp,
                                 T px, T py, T pz, double *acc, int sk0 = 0, int sk1 = NP) {
+    static_assert(!QUAD || sizeof(T) == 4, "the quadratic form is for the FP32 kernel");
+    const T qa = fma(px, px, (T)1), qb = px + px;  // QUAD: (1 + px^2), 2 px
     // FP64 with n > 5: squares formed in the multiply-adds and rows not unrolled
     // (122 instead of 164 VGPRs: FP64 M2L at n = 6 took 307 ms instead of 421 ms at
     // N = 2^22); otherwise precomputed squares and unrolled rows (faster there).
@@ -81,16 +88,27 @@ __device__ inline void m2l_pair(double tx, double ty, double tz, const double *s
             if constexpr (LEAN) rrjk = fma(dy, dy, dz2);
             else rrjk = dy2[sj] + dz2;
             const T prjk = fma(py, dy, pzdz);
+            T qc = 0, qbj = 0;  // QUAD: the row's constant and linear coefficients
+            if constexpr (QUAD) {
+                qc = fma(prjk, prjk, rrjk);
+                qbj = qb * prjk;
+            }
             const V4 *row = smp + NP * (sj + NP * sk);
             T G, Ax, Az;
 #pragma unroll
             for (int si = 0; si < NP; ++si) {
                 const T dx = dxr[si];
-                T rr;
-                if constexpr (LEAN) rr = fma(dx, dx, rrjk);
-                else rr = dx2[si] + rrjk;
-                const T pr = fma(px, dx, prjk);
-                const T y = rsq<T>(fma(pr, pr, rr));
+                T d;
+                if constexpr (QUAD) {
+                    d = fma(dx, fma(qa, dx, qbj), qc);
+                } else {
+                    T rr;
+                    if constexpr (LEAN) rr = fma(dx, dx, rrjk);
+                    else rr = dx2[si] + rrjk;
+                    const T pr = fma(px, dx, prjk);
+                    d = fma(pr, pr, rr);
+                }
+                const T y = rsq<T>(d);
                 const T y3 = (y * y) * y;
                 const V4 m = row[si];
                 const T u = y3 * dx;
@@ -126,7 +144,7 @@ __device__ inline void m2l_pair(double tx, double ty, double tz, const double *s
     acc[5] += (double)(b2 - Uy);
 }
 
-template <int NP, bool MIXED>
+template <int NP, bool MIXED, bool QUAD>
 __global__ __launch_bounds__((M2LShape<NP, MIXED>::BLK)) void k_m2l(
     const int32_t *__restrict__ off, const uint64_t *__restrict__ keys, int sb,
     const double *__restrict__ ncoord, const void *__restrict__ mpv,
@@ -177,11 +195,13 @@ __global__ __launch_bounds__((M2LShape<NP, MIXED>::BLK)) void k_m2l(
             smp = lmp;
         }
         if constexpr (S > 1) {
-            if (part < S) m2l_pair<NP, T, V4>(tx[0], ty[0], tz[0], sn, smp, px, py, pz, acc[0], sk0, sk1);
+            if (part < S)
+                m2l_pair<NP, T, V4, QUAD>(tx[0], ty[0], tz[0], sn, smp, px, py, pz, acc[0], sk0, sk1);
         } else {
 #pragma unroll
             for (int q = 0; q < TPT; ++q)
-                if (tid + q * BLK < NN) m2l_pair<NP, T, V4>(tx[q], ty[q], tz[q], sn, smp, px, py, pz, acc[q]);
+                if (tid + q * BLK < NN)
+                    m2l_pair<NP, T, V4, QUAD>(tx[q], ty[q], tz[q], sn, smp, px, py, pz, acc[q]);
         }
     }
     if constexpr (S > 1) {  // second half into the first, fixed order
@@ -208,17 +228,23 @@ __global__ __launch_bounds__((M2LShape<NP, MIXED>::BLK)) void k_m2l(
 void launch_m2l(Ctx &c, const int32_t *off, const uint64_t *keys, int sb, int64_t ngroups_grid) {
     hipStream_t st = c.stream;
     const bool mixed = c.precision == FMM4RB_MIXED;
+    // the quadratic form in dx needs |px| <= 0.1 (p_avg is on the host since the
+    // input check of the call)
+    const bool quad = mixed && std::fabs(c.pavg_host[0]) <= 0.1;
     const void *mp = mixed ? (const void *)c.d_mp4f.p : (const void *)c.d_mp4.p;
     const unsigned grid = (unsigned)ngroups_grid;
-#define M2L_L(NP)                                                                                \
-    if (mixed)                                                                                   \
-        hipLaunchKernelGGL((k_m2l<NP, true>), dim3(grid), dim3(M2LShape<NP, true>::BLK), 0, st, \
-                           off, keys, sb, c.d_ncoord.as<double>(), mp, c.d_pavg.as<double>(),    \
-                           c.d_loc.as<double>());                                                \
-    else                                                                                         \
-        hipLaunchKernelGGL((k_m2l<NP, false>), dim3(grid), dim3(M2LShape<NP, false>::BLK), 0,   \
-                           st, off, keys, sb, c.d_ncoord.as<double>(), mp,                       \
-                           c.d_pavg.as<double>(), c.d_loc.as<double>());
+#define M2L_K(NP, MX, QD)                                                                        \
+    hipLaunchKernelGGL((k_m2l<NP, MX, QD>), dim3(grid), dim3(M2LShape<NP, MX>::BLK), 0, st, off, \
+                       keys, sb, c.d_ncoord.as<double>(), mp, c.d_pavg.as<double>(),            \
+                       c.d_loc.as<double>())
+#define M2L_L(NP)                    \
+    if (quad) {                      \
+        M2L_K(NP, true, true);       \
+    } else if (mixed) {              \
+        M2L_K(NP, true, false);      \
+    } else {                         \
+        M2L_K(NP, false, false);     \
+    }
     switch (c.np) {
         case 2: M2L_L(2); break;
         case 3: M2L_L(3); break;

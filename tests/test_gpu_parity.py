@@ -148,6 +148,25 @@ def test_mixed_close_to_fp64(ctx, n):
     assert W.rel_l2(Em, E) <= MIXED_TOL and W.rel_l2(Bm, B) <= MIXED_TOL
 
 
+@pytest.mark.parametrize("beam", ["along_z", "px_0.09", "along_x", "diagonal"])
+def test_mixed_close_to_fp64_beam_directions(ctx, beam):
+    """The mixed M2L forms R.R + (p.R)^2 as a quadratic in the x offset when
+    |p_avg,x| <= 0.1 and in the general form otherwise: both within 1e-5 of FP64,
+    including at the bound and for beams moving along x and along a diagonal."""
+    pos, mom = W.bunch(1 << 15, seed=13)
+    if beam == "px_0.09":
+        mom[:, 0] += 0.09
+    elif beam == "along_x":
+        pos, mom = pos[:, ::-1].copy(), mom[:, ::-1].copy()
+    elif beam == "diagonal":
+        rot = np.linalg.qr(np.random.default_rng(14).standard_normal((3, 3)))[0]
+        pos, mom = pos @ rot.T, mom @ rot.T
+    E, B, st = ctx.field(pos, mom, -1.0, 1e-3, 4, 125, 0.5, "fp64")
+    Em, Bm, _ = ctx.field(pos, mom, -1.0, 1e-3, 4, 125, 0.5, "mixed")
+    assert (abs(st.p_avg[0]) <= 0.1) == (beam in ("along_z", "px_0.09"))
+    assert W.rel_l2(Em, E) <= MIXED_TOL and W.rel_l2(Bm, B) <= MIXED_TOL
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_deterministic(ctx, prec):
     pos, mom = W.bunch(1 << 15, seed=9)
