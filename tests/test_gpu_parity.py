@@ -221,9 +221,9 @@ def test_op_m2l_matches_oracle(ctx, orc):
 
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 10])
 def test_op_m2l_mixed_every_order(ctx, orc, n):
-    """Mixed-precision M2L (matrix-core form for n <= 6, vector form above) on
-    admissible pairs: 8 boxes on a grid of spacing 5, sides <= 1.5, all ordered
-    pairs, a relativistic mean momentum; FP64 locals and the oracle agree."""
+    """Mixed-precision M2L at every order (the general form of R.R + (p.R)^2: the
+    mean momentum's x component is 0.3) on admissible pairs: 8 boxes on a grid of
+    spacing 5, sides <= 1.5, all ordered pairs; FP64 locals and the oracle agree."""
     rng = np.random.default_rng(40 + n)
     nc = 8
     cen = np.array([[5.0 * (i % 2), 5.0 * ((i // 2) % 2), 5.0 * (i // 4)] for i in range(nc)])
