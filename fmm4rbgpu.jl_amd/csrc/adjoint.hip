@@ -587,7 +587,7 @@ struct AdjTWaves {
     static constexpr int v = (!MIXED && NP >= 4 && NP <= 6) ? 3 : 1;
 };
 
-template <int NP, bool FWD, bool MIXED>
+template <int NP, bool FWD, bool MIXED, bool QUAD = false>
 __global__ __launch_bounds__(AShape<NP>::BLK)
 __attribute__((amdgpu_waves_per_eu(AdjTWaves<NP, MIXED>::v))) void k_adj_m2l_t(
     const int32_t *__restrict__ off, const uint64_t *__restrict__ keys, int sb,
@@ -616,6 +616,7 @@ __attribute__((amdgpu_waves_per_eu(AdjTWaves<NP, MIXED>::v))) void k_adj_m2l_t(
     const int32_t beg = off[t], end = off[t + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
     const T px = (T)pavg4[0], py = (T)pavg4[1], pz = (T)pavg4[2];
+    const T qa = fma(px, px, (T)1);  // QUAD: 1 + px^2
     double tx[TPT], ty[TPT], tz[TPT];
     T e[TPT][3], b[TPT][3], ac[TPT][NA];
     T u3c[TPT][3];       // e0/3, b1/3, b2/3: (J^T u)_x / 3 per pair for the column sums
@@ -681,6 +682,8 @@ __attribute__((amdgpu_waves_per_eu(AdjTWaves<NP, MIXED>::v))) void k_adj_m2l_t(
                     const T dy = (T)(ty[q] - sn[NP + sj]);
                     const T rrjk = fma(dy, dy, dz * dz) + (T)EPS64;
                     const T prjk = fma(py, dy, pz * dz);
+                    T hj = 0, qc = 0;  // QUAD: px p.R_yz and R_yz.R_yz + (p.R_yz)^2 + eps
+                    if constexpr (QUAD) hj = px * prjk, qc = fma(prjk, prjk, rrjk);
                     const T er = fma(dy, e1, dz * e2);        // e.R - dx e0
                     const T v0 = fma(dy, b2, -(dz * b1));     // (R x b)_0
                     const T v1r = dz * b0, v2r = dy * b0;     // (R x b)_1 + dx b2, (R x b)_2 - dx b1
@@ -688,8 +691,15 @@ __attribute__((amdgpu_waves_per_eu(AdjTWaves<NP, MIXED>::v))) void k_adj_m2l_t(
 #pragma unroll
                     for (int si = 0; si < NP; ++si) {
                         const T dx = dxv[si];
-                        const T pr = fma(px, dx, prjk);
-                        const T d = fma(pr, pr, fma(dx, dx, rrjk));
+                        T d, fx;  // fx = dx + px p.R (the x factor of the coordinate terms)
+                        if constexpr (QUAD) {  // d as a quadratic in dx (as k_m2l with |px| <= 0.1)
+                            fx = fma(qa, dx, hj);
+                            d = fma(dx, fx + hj, qc);
+                        } else {
+                            const T pr = fma(px, dx, prjk);
+                            d = fma(pr, pr, fma(dx, dx, rrjk));
+                            fx = fma(pr, px, dx);
+                        }
                         T y;
                         if constexpr (MIXED) y = __builtin_amdgcn_rsqf(d);
                         else y = rsqrt_nr(d);
@@ -728,7 +738,7 @@ __attribute__((amdgpu_waves_per_eu(AdjTWaves<NP, MIXED>::v))) void k_adj_m2l_t(
                             Uz = fma(uu, m.w, Uz);
                         }
                         if constexpr (YS)  // (J^T u)_x / 3 = y3 u_x / 3 - c (dx + pr px)
-                            Xs[si] = fma(y3, u3, fma(-c, fma(pr, px, dx), Xs[si]));
+                            Xs[si] = fma(y3, u3, fma(-c, fx, Xs[si]));
                     }
                     // pr = px dx + prjk along the row: sum c pr and sum c pr dx from C0..C2
                     const T Cp = fma(px, C1, prjk * C0), Cpx = fma(px, C2, prjk * C1);
@@ -862,7 +872,7 @@ __attribute__((amdgpu_waves_per_eu(AdjTWaves<NP, MIXED>::v))) void k_adj_m2l_t(
 // clusters' local cotangents are staged as FP32).  For NP <= 7 the target side
 // has already summed dl/dY_s per pair (ysp, in transposed-list order): only
 // the multipole cotangents are evaluated here, with 8 sums per node pair.
-template <int NP, bool MIXED>
+template <int NP, bool MIXED, bool QUAD = false>
 __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
     const int32_t *__restrict__ offT, const uint64_t *__restrict__ keysT, int sb,
     const double *__restrict__ ncoord, const void *__restrict__ mpv,
@@ -887,6 +897,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
     const int32_t beg = offT[s], end = offT[s + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
     const T px = (T)pavg4[0], py = (T)pavg4[1], pz = (T)pavg4[2];
+    const T qa = fma(px, px, (T)1), qb = px + px;  // QUAD: 1 + px^2, 2 px
     double sx[TPT], sy[TPT], sz[TPT];
     T ac[TPT][NA];
     double ad[TPT][NA];
@@ -931,6 +942,8 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
                     const T dy = (T)(tn[NP + tj] - sy[q]);
                     const T rrjk = fma(dy, dy, fma(dz, dz, (T)EPS64));
                     const T prjk = fma(py, dy, pz * dz);
+                    T qbj = 0, qc = 0;  // QUAD: 2 px p.R_yz and R_yz.R_yz + (p.R_yz)^2 + eps
+                    if constexpr (QUAD) qbj = qb * prjk, qc = fma(prjk, prjk, rrjk);
                     const T w0 = fma(m.z, dz, -(m.w * dy));  // (m x R)_0
                     const T w1r = -(m.y * dz), w2r = m.y * dy;
                     T Sxe = 0, Ex = 0, Ey = 0, Ez = 0, Bx = 0, By = 0, Bz = 0, Sxby = 0, Sxbz = 0;
@@ -938,8 +951,13 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
 #pragma unroll
                     for (int ti = 0; ti < NP; ++ti) {
                         const T dx = dxv[ti];
-                        const T pr = fma(px, dx, prjk);
-                        const T d = fma(pr, pr, fma(dx, dx, rrjk));
+                        T d;
+                        if constexpr (QUAD) {
+                            d = fma(dx, fma(qa, dx, qbj), qc);
+                        } else {
+                            const T pr = fma(px, dx, prjk);
+                            d = fma(pr, pr, fma(dx, dx, rrjk));
+                        }
                         T y;
                         if constexpr (MIXED) y = __builtin_amdgcn_rsqf(d);
                         else y = rsqrt_nr(d);
@@ -1557,17 +1575,21 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     // 2. forward M2L fused with the M2L^T target side, then the forward L2L; with
     //    saved_locals the final forward locals of the preceding field call are in
     //    loc already and only the M2L^T target side runs
-#define ADJ_M2LT_F(NP, FWD)                                                                        \
-    if (mixed)                                                                                     \
-        hipLaunchKernelGGL((k_adj_m2l_t<NP, FWD, true>), dim3((unsigned)nc),                       \
-                           dim3(AShape<NP>::BLK), 0, st, c.d_m2l_off.as<int32_t>(), c.m2l_sorted,  \
-                           c.sb, ncoord, mpv, lbar, c.d_pavg.as<double>(), xbar, pbar, loc, posT, \
-                           ysp);                                                                   \
-    else                                                                                           \
-        hipLaunchKernelGGL((k_adj_m2l_t<NP, FWD, false>), dim3((unsigned)nc),                      \
-                           dim3(AShape<NP>::BLK), 0, st, c.d_m2l_off.as<int32_t>(), c.m2l_sorted,  \
-                           c.sb, ncoord, mpv, lbar, c.d_pavg.as<double>(), xbar, pbar, loc, posT, \
-                           ysp)
+    // the quadratic form of d in dx (|px| <= 0.1, as the forward M2L) for n <= 6
+    const bool quad = std::fabs(c.pavg_host[0]) <= 0.1;
+#define ADJ_M2LT_K(NP, FWD, MX, QD)                                                                \
+    hipLaunchKernelGGL((k_adj_m2l_t<NP, FWD, MX, QD>), dim3((unsigned)nc), dim3(AShape<NP>::BLK), \
+                       0, st, c.d_m2l_off.as<int32_t>(), c.m2l_sorted, c.sb, ncoord, mpv, lbar,    \
+                       c.d_pavg.as<double>(), xbar, pbar, loc, posT, ysp)
+#define ADJ_M2LT_F(NP, FWD)                          \
+    if (mixed && quad)                               \
+        ADJ_M2LT_K(NP, FWD, true, (NP <= 7));        \
+    else if (mixed)                                  \
+        ADJ_M2LT_K(NP, FWD, true, false);            \
+    else if (quad)                                   \
+        ADJ_M2LT_K(NP, FWD, false, (NP <= 7));       \
+    else                                             \
+        ADJ_M2LT_K(NP, FWD, false, false)
 #define ADJ_M2LT(NP) ADJ_M2LT_F(NP, true)
 #define ADJ_M2LT_SAVED(NP) ADJ_M2LT_F(NP, false)
     if (saved_locals) {
@@ -1611,15 +1633,19 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     FMM_LAUNCH_CHECK();
     FMM_HIP(hipEventRecord(c.evy[1], sg));
     // 4. M2L^T source side
-#define ADJ_M2LS(NP)                                                                              \
-    if (mixed)                                                                                    \
-        hipLaunchKernelGGL((k_adj_m2l_s<NP, true>), dim3((unsigned)nc), dim3(AShape<NP>::BLK), 0, \
-                           st, c.d_m2lT_off.as<int32_t>(), c.m2lT_sorted, c.sb, ncoord, mpv, lbar, \
-                           c.d_pavg.as<double>(), mbar, xbar2, ysp);                              \
-    else                                                                                          \
-        hipLaunchKernelGGL((k_adj_m2l_s<NP, false>), dim3((unsigned)nc), dim3(AShape<NP>::BLK),   \
-                           0, st, c.d_m2lT_off.as<int32_t>(), c.m2lT_sorted, c.sb, ncoord, mpv,    \
-                           lbar, c.d_pavg.as<double>(), mbar, xbar2, ysp)
+#define ADJ_M2LS_K(NP, MX, QD)                                                                    \
+    hipLaunchKernelGGL((k_adj_m2l_s<NP, MX, QD>), dim3((unsigned)nc), dim3(AShape<NP>::BLK), 0, st, \
+                       c.d_m2lT_off.as<int32_t>(), c.m2lT_sorted, c.sb, ncoord, mpv, lbar,          \
+                       c.d_pavg.as<double>(), mbar, xbar2, ysp)
+#define ADJ_M2LS(NP)                        \
+    if (mixed && quad)                      \
+        ADJ_M2LS_K(NP, true, (NP <= 7));    \
+    else if (mixed)                         \
+        ADJ_M2LS_K(NP, true, false);        \
+    else if (quad)                          \
+        ADJ_M2LS_K(NP, false, (NP <= 7));   \
+    else                                    \
+        ADJ_M2LS_K(NP, false, false)
     ADJ_NP_SWITCH(c.np, ADJ_M2LS);
     FMM_LAUNCH_CHECK();
     FMM_HIP(hipEventRecord(c.ev[9], st));
