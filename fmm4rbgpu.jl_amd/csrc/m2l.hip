@@ -12,7 +12,7 @@
 // reads broadcasts; the FP64 kernel reads them with wave-uniform addresses,
 // which become scalar loads (the scalar cache instead of LDS bandwidth).  The node grid is a tensor product, so for a fixed
 // source row (j, k) the y/z parts of R.R and p.R are hoisted and the i loop
-// needs three operations of geometry per evaluation; E and B are accumulated
+// needs three operations of geometry per evaluation (two in the quadratic form); E and B are accumulated
 // through the row sums G = sum y3 g, H = sum y3 dx g, A = sum y3 m,
 // U = sum y3 dx m (y3 = d^-3/2), which cost 9 multiply-adds per evaluation
 // instead of 12.  In the mixed-precision form the row geometry is formed in
@@ -51,7 +51,8 @@ template <>
 __device__ inline float rsq<float>(float d) { return __builtin_amdgcn_rsqf(d); }
 
 // One source cluster's contribution to one target node.  Per node pair: the
-// geometry (3 operations on row-hoisted terms), d^-1/2, y3 and 8 products/sums.
+// geometry (3 operations on row-hoisted terms; QUAD: 2), d^-1/2, y3 and 8
+// products/sums.
 // Running sums over the whole source cluster: H (E_x), U_y, U_z; per plane (dz
 // fixed): G, A_x, A_y; per row (dy fixed): G, A_x, A_z.  So E = (H, sum_rows dy G,
 // sum_planes dz G), B = (sum dz A_y - sum dy A_z, U_z - sum dz A_x, sum dy A_x - U_y).
