@@ -31,6 +31,99 @@
 #define EPS64 DBL_EPSILON /* Julia eps() / eps(Float64) */
 static const double AMP_RE = 2.8179403699772166e-15; /* update_particles_field.jl:28 */
 
+/* ------------------------------------------------------------- threading */
+
+/* The operators below run their outer loops on OpenMP threads over groups
+ * that own disjoint outputs (a target cluster, a parent, a leaf), each group
+ * visiting its inputs in the serial order: every output receives the same
+ * additions in the same order as in the serial loops, so results are
+ * bit-identical for any thread count.  When a call's outputs are not disjoint
+ * between groups (overlapping particle ranges, a node listed with its
+ * parent), the loop runs serially. */
+static int g_threads = 0; /* 0: all the runtime's threads */
+
+void orc_set_threads(int n) { g_threads = n < 0 ? 0 : n; }
+
+static int nthreads_for(int64_t work) {
+#ifdef _OPENMP
+    int t = g_threads > 0 ? g_threads : omp_get_max_threads();
+    if (work < 2) t = 1;
+    return t;
+#else
+    (void)work;
+    return 1;
+#endif
+}
+
+typedef struct {
+    int64_t key, pos;
+} keypos;
+
+static int cmp_keypos(const void *a, const void *b) {
+    const keypos *x = (const keypos *)a, *y = (const keypos *)b;
+    if (x->key != y->key) return (x->key > y->key) - (x->key < y->key);
+    return (x->pos > y->pos) - (x->pos < y->pos);
+}
+
+/* Stable grouping of n items by key: order[] lists item positions grouped by
+ * key, each group in the items' original order; goff[g..g+1] bounds group g.
+ * Returns the number of groups (-1 if out of memory). */
+static int64_t group_by_key(int64_t n, const int64_t *keys, int64_t stride, int64_t **order,
+                            int64_t **goff) {
+    keypos *kp = malloc(sizeof(keypos) * (size_t)(n > 0 ? n : 1));
+    *order = malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    *goff = malloc(sizeof(int64_t) * (size_t)(n + 1));
+    if (!kp || !*order || !*goff) {
+        free(kp); free(*order); free(*goff);
+        *order = *goff = NULL;
+        return -1;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        kp[i].key = keys[i * stride];
+        kp[i].pos = i;
+    }
+    qsort(kp, (size_t)n, sizeof(keypos), cmp_keypos);
+    int64_t ng = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        (*order)[i] = kp[i].pos;
+        if (i == 0 || kp[i].key != kp[i - 1].key) (*goff)[ng++] = i;
+    }
+    (*goff)[ng] = n;
+    free(kp);
+    return ng;
+}
+
+/* 1 if the particle ranges [lo[k], hi[k]] of the listed nodes are pairwise
+ * disjoint (a node listed twice counts as overlapping). */
+static int ranges_disjoint(int64_t n, const int64_t *nodes, const int64_t *lo, const int64_t *hi) {
+    int64_t *order = NULL, *goff = NULL;
+    int64_t *l = malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    if (!l) return 0;
+    for (int64_t a = 0; a < n; ++a) l[a] = lo[nodes[a]];
+    int64_t ng = group_by_key(n, l, 1, &order, &goff);
+    int ok = ng == n;
+    for (int64_t g = 1; ok && g < ng; ++g)
+        ok = lo[nodes[order[goff[g]]]] > hi[nodes[order[goff[g - 1]]]];
+    free(l); free(order); free(goff);
+    return ok;
+}
+
+/* 1 if no listed node is the parent of another listed node. */
+static int no_parent_listed(int64_t n, const int64_t *nodes, const int64_t *parents) {
+    int64_t mx = 0;
+    for (int64_t a = 0; a < n; ++a) mx = nodes[a] > mx ? nodes[a] : mx;
+    uint8_t *in = calloc((size_t)mx + 1, 1);
+    if (!in) return 0;
+    for (int64_t a = 0; a < n; ++a) in[nodes[a]] = 1;
+    int ok = 1;
+    for (int64_t a = 0; ok && a < n; ++a) {
+        int64_t p = parents[nodes[a]];
+        ok = !(p >= 0 && p <= mx && in[p]);
+    }
+    free(in);
+    return ok;
+}
+
 /* ---------------------------------------------------------------- treeinfo */
 
 /* treeinfo.jl:6-8: ceil(log(N/N0)/log(2)) if N > N0 else 0 */
@@ -330,8 +423,11 @@ void orc_p2m(int n, int rule, int64_t nnodes, const int64_t *nodes, const int64_
              const int64_t *lo, const int64_t *hi, const double *bboxes, const double *pos,
              const double *mom, double *gam, double *mmom) {
     int np = n + 1;
-    double xc[11], yc[11], zc[11], wx[11], wy[11], wz[11];
+    /* each listed node owns its expansion: nodes in parallel when distinct */
+    int par = ranges_disjoint(nnodes, nodes, lo, hi);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(par ? nthreads_for(nnodes) : 1)
     for (int64_t a = 0; a < nnodes; ++a) {
+        double xc[11], yc[11], zc[11], wx[11], wy[11], wz[11];
         int64_t node = nodes[a];
         const double *bb = bboxes + 6 * node;
         orc_cheb2(n, bb[0], bb[3], xc);
@@ -360,10 +456,23 @@ void orc_p2m(int n, int rule, int64_t nnodes, const int64_t *nodes, const int64_
 void orc_m2m(int n, int rule, int64_t nnodes, const int64_t *nodes, const int64_t *parents,
              const double *bboxes, double *gam, double *mmom) {
     int np = n + 1;
-    double cx[11], cy[11], cz[11], px[11], py[11], pz[11], wx[11], wy[11], wz[11];
-    for (int64_t a = 0; a < nnodes; ++a) {
-        int64_t node = nodes[a], par = parents[node];
-        const double *cb = bboxes + 6 * node, *pb = bboxes + 6 * par;
+    /* the nodes grouped by parent, each group in list order: one parent's
+     * additions come from its group only, in the serial order */
+    int64_t *order = NULL, *goff = NULL, *pk = malloc(sizeof(int64_t) * (size_t)(nnodes + 1));
+    int64_t ng = -1;
+    if (pk) {
+        for (int64_t a = 0; a < nnodes; ++a) pk[a] = parents[nodes[a]];
+        ng = group_by_key(nnodes, pk, 1, &order, &goff);
+    }
+    int par = ng >= 0 && no_parent_listed(nnodes, nodes, parents);
+    if (!par) ng = 1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(par ? nthreads_for(ng) : 1)
+    for (int64_t g = 0; g < ng; ++g)
+    for (int64_t e = par ? goff[g] : 0; e < (par ? goff[g + 1] : nnodes); ++e) {
+        double cx[11], cy[11], cz[11], px[11], py[11], pz[11], wx[11], wy[11], wz[11];
+        int64_t a = par ? order[e] : e;
+        int64_t node = nodes[a], pnode = parents[node];
+        const double *cb = bboxes + 6 * node, *pb = bboxes + 6 * pnode;
         orc_cheb2(n, cb[0], cb[3], cx);
         orc_cheb2(n, cb[1], cb[4], cy);
         orc_cheb2(n, cb[2], cb[5], cz);
@@ -382,7 +491,7 @@ void orc_m2m(int n, int rule, int64_t nnodes, const int64_t *nodes, const int64_
                         for (int pj = 0; pj < np; ++pj)
                             for (int pi = 0; pi < np; ++pi) {
                                 double w = (wx[pi] * wy[pj]) * wz[pk];
-                                int64_t pid = IDX(pi, pj, pk, par, np);
+                                int64_t pid = IDX(pi, pj, pk, pnode, np);
                                 gam[pid] += cg * w;
                                 for (int d = 0; d < 3; ++d) mmom[3 * pid + d] += cm[d] * w;
                             }
@@ -390,6 +499,7 @@ void orc_m2m(int n, int rule, int64_t nnodes, const int64_t *nodes, const int64_
             }
         }
     }
+    free(pk); free(order); free(goff);
 }
 
 /* K(R; p) = R / sqrt(R.R + (p.R)^2 + eps)^3 (interaction.jl:36, :69) */
@@ -412,8 +522,17 @@ void orc_m2l(int n, int64_t npairs, const int64_t *pairs, const double *bboxes,
              const double p_avg[3], const double *gam, const double *mmom, double *le,
              double *lb) {
     int np = n + 1;
-    double sx[11], sy[11], sz[11], tx[11], ty[11], tz[11];
-    for (int64_t a = 0; a < npairs; ++a) {
+    /* pairs grouped by target, each group in list order (a target's
+     * expansion receives the same additions in the same order as serially) */
+    int64_t *order = NULL, *goff = NULL;
+    int64_t ng = group_by_key(npairs, pairs, 2, &order, &goff);
+    int par = ng >= 0;
+    if (!par) ng = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(par ? nthreads_for(ng) : 1)
+    for (int64_t g = 0; g < ng; ++g)
+    for (int64_t e = par ? goff[g] : 0; e < (par ? goff[g + 1] : npairs); ++e) {
+        double sx[11], sy[11], sz[11], tx[11], ty[11], tz[11];
+        int64_t a = par ? order[e] : e;
         int64_t t = pairs[2 * a], s = pairs[2 * a + 1];
         const double *sb = bboxes + 6 * s, *tb = bboxes + 6 * t;
         orc_cheb2(n, sb[0], sb[3], sx);
@@ -438,12 +557,31 @@ void orc_m2l(int n, int64_t npairs, const int64_t *pairs, const double *bboxes,
                             }
                 }
     }
+    free(order); free(goff);
 }
 
 /* interaction.jl:49-75 */
 void orc_p2p(int64_t npairs, const int64_t *pairs, const int64_t *pind, const int64_t *lo,
              const int64_t *hi, const double *pos, const double *mom, double *E, double *B) {
-    for (int64_t a = 0; a < npairs; ++a) {
+    /* pairs grouped by target cluster, each group in list order; in parallel
+     * when the target clusters' particle ranges are disjoint */
+    int64_t *order = NULL, *goff = NULL;
+    int64_t ng = group_by_key(npairs, pairs, 2, &order, &goff);
+    int par = ng >= 0;
+    if (par) {
+        int64_t *tg = malloc(sizeof(int64_t) * (size_t)(ng > 0 ? ng : 1));
+        par = tg != NULL;
+        if (par) {
+            for (int64_t g = 0; g < ng; ++g) tg[g] = pairs[2 * order[goff[g]]];
+            par = ranges_disjoint(ng, tg, lo, hi);
+            free(tg);
+        }
+    }
+    if (!par) ng = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(par ? nthreads_for(ng) : 1)
+    for (int64_t g = 0; g < ng; ++g)
+    for (int64_t e = par ? goff[g] : 0; e < (par ? goff[g + 1] : npairs); ++e) {
+        int64_t a = par ? order[e] : e;
         int64_t t = pairs[2 * a], s = pairs[2 * a + 1];
         for (int64_t i = lo[t]; i <= hi[t]; ++i) {
             int64_t pi = pind[i];
@@ -459,16 +597,24 @@ void orc_p2p(int64_t npairs, const int64_t *pairs, const int64_t *pind, const in
             }
         }
     }
+    free(order); free(goff);
 }
 
 /* downwardpass.jl:3-45 */
 void orc_l2l(int n, int rule, int64_t nnodes, const int64_t *nodes, const int64_t *parents,
              const double *bboxes, double *le, double *lb) {
     int np = n + 1;
-    double cx[11], cy[11], cz[11], px[11], py[11], pz[11], wx[11], wy[11], wz[11];
+    /* each node owns its expansion and reads its parent's: nodes in parallel
+     * when they are distinct and no parent is listed */
+    int64_t *order = NULL, *goff = NULL;
+    int64_t ng = group_by_key(nnodes, nodes, 1, &order, &goff);
+    int par = ng == nnodes && no_parent_listed(nnodes, nodes, parents);
+    free(order); free(goff);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(par ? nthreads_for(nnodes) : 1)
     for (int64_t a = 0; a < nnodes; ++a) {
-        int64_t node = nodes[a], par = parents[node];
-        const double *cb = bboxes + 6 * node, *pb = bboxes + 6 * par;
+        double cx[11], cy[11], cz[11], px[11], py[11], pz[11], wx[11], wy[11], wz[11];
+        int64_t node = nodes[a], pnode = parents[node];
+        const double *cb = bboxes + 6 * node, *pb = bboxes + 6 * pnode;
         orc_cheb2(n, cb[0], cb[3], cx);
         orc_cheb2(n, cb[1], cb[4], cy);
         orc_cheb2(n, cb[2], cb[5], cz);
@@ -486,7 +632,7 @@ void orc_l2l(int n, int rule, int64_t nnodes, const int64_t *nodes, const int64_
                         for (int pj = 0; pj < np; ++pj)
                             for (int pi = 0; pi < np; ++pi) {
                                 double w = (wx[pi] * wy[pj]) * wz[pk];
-                                int64_t pid = IDX(pi, pj, pk, par, np);
+                                int64_t pid = IDX(pi, pj, pk, pnode, np);
                                 for (int d = 0; d < 3; ++d) {
                                     le[3 * cid + d] += le[3 * pid + d] * w;
                                     lb[3 * cid + d] += lb[3 * pid + d] * w;
@@ -503,8 +649,11 @@ void orc_l2p(int n, int rule, int64_t nnodes, const int64_t *nodes, const int64_
              const int64_t *lo, const int64_t *hi, const double *bboxes, const double *pos,
              const double *le, const double *lb, double *E, double *B) {
     int np = n + 1;
-    double xc[11], yc[11], zc[11], wx[11], wy[11], wz[11];
+    /* each node owns its particles' fields: in parallel when ranges are disjoint */
+    int par = ranges_disjoint(nnodes, nodes, lo, hi);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(par ? nthreads_for(nnodes) : 1)
     for (int64_t a = 0; a < nnodes; ++a) {
+        double xc[11], yc[11], zc[11], wx[11], wy[11], wz[11];
         int64_t node = nodes[a];
         const double *bb = bboxes + 6 * node;
         orc_cheb2(n, bb[0], bb[3], xc);

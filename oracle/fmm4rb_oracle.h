@@ -91,6 +91,11 @@ void orc_l2p(int n, int rule, int64_t nnodes, const int64_t *nodes, const int64_
              const int64_t *lo, const int64_t *hi, const double *bboxes, const double *pos,
              const double *le, const double *lb, double *E, double *B);
 
+/* Threads for the operators' outer loops (0 = the OpenMP default).  Results
+ * are bit-identical for every thread count: groups own disjoint outputs and
+ * keep the serial order of additions. */
+void orc_set_threads(int n);
+
 /* The serial FMM driver (src/update_particles_field.jl:40-64).  E/B are
  * overwritten (the reference accumulates into whatever they hold).  If
  * p_avg_in is non-NULL it replaces the computed mean momentum.  Returns 0. */

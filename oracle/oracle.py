@@ -83,6 +83,8 @@ def lib():
         L.orc_fmm.restype = C.c_int
         L.orc_fmm.argtypes = [_i64, _dp, _dp, C.c_double, C.c_double, C.c_int, _i64, C.c_double,
                               C.c_void_p, C.c_int, _dp, _dp]
+        L.orc_set_threads.restype = None
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_bruteforce.restype = None
         L.orc_bruteforce.argtypes = [_i64, _dp, _dp, C.c_double, C.c_double, _i64, C.c_void_p,
                                      C.c_int, _dp, _dp]
@@ -265,6 +267,11 @@ def l2p(n, tree, pos, le, lb, nodes=None, rule=SINGULAR_LAST, E=None, B=None):
 
 
 # --------------------------------------------------------------------- drivers
+def set_threads(n: int) -> None:
+    """Threads for the operators (0 = all); results do not depend on it."""
+    lib().orc_set_threads(int(n))
+
+
 def fmm(pos, mom, charge=-1.0, lam=1.0, n=4, N0=125, eta=0.5, pavg=None, rule=SINGULAR_LAST):
     pos, mom = _f64(pos), _f64(mom)
     N = pos.shape[0]

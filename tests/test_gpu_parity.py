@@ -285,3 +285,18 @@ def test_errors(ctx, bad):
     assert ei.value.code == 1
     E, B, _ = ctx.field(*W.bunch(500, seed=13), -1.0, 1e-3, 4, 125, 0.5)  # context still usable
     assert np.all(np.isfinite(E))
+
+
+@pytest.mark.parametrize("name", ["A", "B"])
+def test_lists_hand_derived_kat(ctx, name):
+    # the GPU tree and traversal on the particles of a hand-derived traversal case
+    # (tests/traversal_kats.py, from interaction_lists.jl:23-49): the tree is the
+    # hand-built one and the lists are the hand-derived ones
+    import traversal_kats as TK
+    case = TK.CASES[name]
+    g = ctx.tree(case["pos"], case["mom"], case["N0"])
+    assert np.array_equal(g["children"], TK.CHILDREN)
+    assert np.array_equal(g["bboxes"], case["bboxes"])
+    gp2p, gm2l = ctx.lists(case["pos"], case["mom"], case["N0"], case["eta"])
+    assert np.array_equal(gp2p, TK.sorted_pairs(case["p2p"]))
+    assert np.array_equal(gm2l, TK.sorted_pairs(case["m2l"]))

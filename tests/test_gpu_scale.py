@@ -15,6 +15,35 @@ def bunch20():
     return W.bunch(1 << 20, seed=2)
 
 
+def test_oracle_parity_2e20(ctx, orc, bunch20):
+    """The N=2^20 bunch (n=4, N0=125, eta=0.5) against the oracle: tree
+    (permutation, ranges, children, bounding-box bits) and both interaction lists
+    bit-exact, FP64 fields within 1e-12 of the oracle's FMM.  The oracle runs on
+    all host threads (bit-identical to its serial loops) and gets the GPU's p_avg."""
+    import time
+    pos, mom = bunch20
+    N0, n, eta, lam = 125, 4, 0.5, 1e-3
+    g = ctx.tree(pos, mom, N0)
+    s = orc.stretch_of(g["p_avg"])
+    t0 = time.time()
+    o = orc.build_tree(pos, N0, s)
+    assert np.array_equal(g["parindices"], o["parindices"])
+    assert np.array_equal(g["lo"], o["lo"]) and np.array_equal(g["hi"], o["hi"])
+    assert np.array_equal(g["children"], o["children"])
+    assert np.array_equal(g["bboxes"].view(np.uint64), o["bboxes"].view(np.uint64))
+    op2p, om2l = orc.lists(o, s, eta)
+    gp2p, gm2l = ctx.lists(pos, mom, N0, eta)
+    key = lambda a: a[np.lexsort((a[:, 1], a[:, 0]))]
+    assert np.array_equal(gp2p, key(op2p)) and np.array_equal(gm2l, key(om2l))
+    E, B, st = ctx.field(pos, mom, -1.0, lam, n, N0, eta, "fp64")
+    t1 = time.time()
+    Eo, Bo = orc.fmm(pos, mom, -1.0, lam, n, N0, eta, pavg=np.array(st.p_avg))
+    ee, eb = W.rel_l2(E, Eo), W.rel_l2(B, Bo)
+    print(f"2^20 oracle parity: np2p {len(gp2p)} nm2l {len(gm2l)}; fields E {ee:.2e} B {eb:.2e}; "
+          f"oracle tree+lists {t1 - t0:.1f} s, FMM {time.time() - t1:.1f} s")
+    assert ee <= 1e-12 and eb <= 1e-12
+
+
 def test_tree_invariants_2e20(ctx, bunch20):
     pos, mom = bunch20
     N, N0 = len(pos), 125
@@ -58,7 +87,9 @@ def test_accuracy_and_precision_2e20(ctx, bunch20, n):
     Eb, Bb = ctx.bruteforce(pos, mom, -1.0, 1e-3, targets=tg)
     E, B, st = ctx.field(pos, mom, -1.0, 1e-3, n, 125, 0.5, "fp64")
     Em, Bm, _ = ctx.field(pos, mom, -1.0, 1e-3, n, 125, 0.5, "mixed")
-    assert W.rel_l2(E[tg], Eb) < 1e-2 and W.rel_l2(B[tg], Bb) < 1e-2
+    # truncation error of the expansion, observed 1.75e-3 (n=4) and 3.0e-4 (n=6)
+    bar = {4: 2.5e-3, 6: 4e-4}[n]
+    assert W.rel_l2(E[tg], Eb) <= bar and W.rel_l2(B[tg], Bb) <= bar
     assert W.rel_l2(Em, E) <= 1e-5 and W.rel_l2(Bm, B) <= 1e-5
     print(f"n={n} fp64-vs-BF E {W.rel_l2(E[tg], Eb):.3e} B {W.rel_l2(B[tg], Bb):.3e}; "
           f"mixed-vs-fp64 E {W.rel_l2(Em, E):.3e} B {W.rel_l2(Bm, B):.3e}; "
@@ -98,23 +129,34 @@ def test_adjoint_fd_2e20(ctx, bunch20):
     errs = np.array(errs)
     print(f"2^20 adjoint: {len(errs)} coords, median {np.median(errs):.2e}, max {errs.max():.2e}, "
           f"skipped {skipped}; forward {fst.ms_total:.1f} ms, vjp {st.ms_total:.1f} ms")
-    assert skipped <= 16 and np.median(errs) <= 1e-5 and errs.max() <= 1e-2
+    # observed: median 3.8e-7, max 3.7e-5, no coordinate skipped
+    assert skipped <= 4 and np.median(errs) <= 1e-6 and errs.max() <= 1e-4
 
 
-def test_bunch_2e22_n6(ctx):
-    """N=2^22 bunch with n=6: mixed within 1e-5 of FP64; FP64 against GPU
-    BruteForce on 4096 random targets."""
+@pytest.fixture(scope="module")
+def bunch22():
     pos, mom = W.bunch(1 << 22, seed=3)
-    tg = np.random.default_rng(41).choice(len(pos), 4096, replace=False)
-    E, B, st = ctx.field(pos, mom, -1.0, 1e-3, 6, 125, 0.5, "fp64")
-    Em, Bm, sm = ctx.field(pos, mom, -1.0, 1e-3, 6, 125, 0.5, "mixed")
+    return pos, mom, np.random.default_rng(41).choice(len(pos), 4096, replace=False)
+
+
+@pytest.mark.parametrize("n", [4, 6])
+def test_bunch_2e22(ctx, bunch22, n):
+    """N=2^22 bunch (N0=125, eta=0.5), the headline configuration at n=4 and its
+    n=6 variant: mixed within 1e-5 of FP64; FP64 and mixed against GPU BruteForce
+    on 4096 random targets."""
+    pos, mom, tg = bunch22
+    E, B, st = ctx.field(pos, mom, -1.0, 1e-3, n, 125, 0.5, "fp64")
+    Em, Bm, sm = ctx.field(pos, mom, -1.0, 1e-3, n, 125, 0.5, "mixed")
     Eb, Bb = ctx.bruteforce(pos, mom, -1.0, 1e-3, targets=tg)
     ef, bf = W.rel_l2(E[tg], Eb), W.rel_l2(B[tg], Bb)
+    mf = W.rel_l2(Em[tg], Eb)
     em, bm = W.rel_l2(Em, E), W.rel_l2(Bm, B)
-    print(f"2^22 n=6: fp64-vs-BF E {ef:.3e} B {bf:.3e}; mixed-vs-fp64 E {em:.3e} B {bm:.3e}; "
-          f"ms fp64 {st.ms_total:.1f} mixed {sm.ms_total:.1f}")
+    print(f"2^22 n={n}: fp64-vs-BF E {ef:.3e} B {bf:.3e}, mixed-vs-BF E {mf:.3e}; "
+          f"mixed-vs-fp64 E {em:.3e} B {bm:.3e}; ms fp64 {st.ms_total:.1f} mixed {sm.ms_total:.1f}")
     assert em <= 1e-5 and bm <= 1e-5
-    assert ef <= 1e-3 and bf <= 1e-3
+    # observed 4.76e-3 (n=4) and 3.65e-4 (n=6) on these 4096 targets
+    bar = {4: 5e-3, 6: 5e-4}[n]
+    assert ef <= bar and bf <= bar and mf <= bar
 
 
 def test_ellipsoid_25_6M(ctx):
@@ -141,6 +183,7 @@ def test_ellipsoid_25_6M(ctx):
           f"B {bm:.3e}; ms fp64 {st.ms_total:.1f} mixed {sm.ms_total:.1f} vjp {sa.ms_total:.1f} "
           f"(n=6 fp64 {s6.ms_total:.1f}); leaves {st.nleaf} nm2l {st.nm2l} np2p {st.np2p}")
     assert em <= 1e-5 and bm <= 1e-5
-    assert ef <= 5e-2 and bf <= 5e-2 and e6 <= ef / 3
+    # the interpolation error of a gamma=200 beam: observed 1.9e-2 (n=4), 1.05e-3 (n=6)
+    assert ef <= 2.5e-2 and bf <= 2.5e-2 and e6 <= 1.5e-3 and e6 <= ef / 3
     assert np.all(np.isfinite(gx)) and np.all(np.isfinite(gp))
     assert np.abs(gx).max() > 0 and np.abs(gp).max() > 0

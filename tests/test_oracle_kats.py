@@ -158,3 +158,38 @@ def test_p2p_far_field(orc):
     assert isapprox(B[0], [0, 0, 0], rtol=1e-3)
     assert isapprox(E[100], far, rtol=1e-3)
     assert isapprox(B[100], [0, 0, 0], rtol=1e-3)
+
+
+# ----------------------------------------------------------------------------
+# Dual traversal: lists derived by hand from interaction_lists.jl:23-49 on two
+# 7-node trees (tests/traversal_kats.py holds the trees and the derivation).
+import traversal_kats as TK  # noqa: E402
+
+
+def test_traversal_kat_hand_tree(orc):
+    for name, case in TK.CASES.items():
+        p2p, m2l = orc.lists(TK.hand_tree(case), case["stretch"], case["eta"])
+        # the reference's push order (depth first, first child first)
+        assert [tuple(p) for p in p2p] == case["p2p"], name
+        assert [tuple(p) for p in m2l] == case["m2l"], name
+
+
+def test_traversal_kat_tree_from_particles(orc):
+    # the median-split tree of the case's particles is the hand-built tree
+    for name, case in TK.CASES.items():
+        s = orc.stretch_of(orc.p_avg(case["mom"]))
+        assert np.array_equal(s, case["stretch"]), name
+        t = orc.build_tree(case["pos"], case["N0"], s)
+        assert np.array_equal(t["children"], TK.CHILDREN), name
+        assert np.array_equal(t["bboxes"], case["bboxes"]), name
+        p2p, m2l = orc.lists(t, s, case["eta"])
+        assert [tuple(p) for p in p2p] == case["p2p"], name
+        assert [tuple(p) for p in m2l] == case["m2l"], name
+
+
+def test_traversal_kat_stretch_decides_split(orc):
+    # case B without the stretch: node 2's diameter (.9) is below node 1's (1), so
+    # the reference splits the source at (2, 1) and the lists change
+    case = TK.CASE_B
+    p2p, m2l = orc.lists(TK.hand_tree(case), [1.0, 1.0, 1.0], case["eta"])
+    assert [tuple(p) for p in m2l] != case["m2l"] or [tuple(p) for p in p2p] != case["p2p"]
