@@ -161,7 +161,6 @@ static void prepare(Ctx &c, int64_t N, int64_t N0, int n, int precision) {
     c.d_keys_alt.ensure(n3 * 8);
     c.d_vals.ensure(n3 * 4);
     c.d_vals_alt.ensure(n3 * 4);
-    c.d_rs_scratch.ensure(4 * radix_scratch_words(N, 3));
     c.d_seg.ensure((size_t)N * 4);
     c.d_seg_alt.ensure((size_t)N * 4);
     c.d_rk.ensure(2 * 3 * n3 * 4);
@@ -176,11 +175,15 @@ static void prepare(Ctx &c, int64_t N, int64_t N0, int n, int precision) {
     c.d_ncoord.ensure(3 * (size_t)c.np * nc * 8);
     c.d_sp8.ensure(8 * (size_t)N * 8);
     if (precision == FMM4RB_MIXED) c.d_sp8f.ensure(4 * (size_t)N * 4);
-    if (c.cap_front == 0) {
-        c.cap_front = std::max<int64_t>(1 << 16, 48 * nleaf);
-        c.cap_p2p = std::max<int64_t>(1 << 16, 64 * nleaf);
-        c.cap_m2l = std::max<int64_t>(1 << 16, 64 * nleaf);
-    }
+    // list capacities from the leaf count on every call (a context first used at a
+    // small N must not rely on the traversal's overflow retries at a large one)
+    c.cap_front = std::max<int64_t>(c.cap_front, std::max<int64_t>(1 << 16, 48 * nleaf));
+    c.cap_p2p = std::max<int64_t>(c.cap_p2p, std::max<int64_t>(1 << 16, 64 * nleaf));
+    c.cap_m2l = std::max<int64_t>(c.cap_m2l, std::max<int64_t>(1 << 16, 64 * nleaf));
+    // the sort scratch also covers the pair sorts of lists up to their capacity
+    c.d_rs_scratch.ensure(
+        4 * std::max(radix_scratch_words(N, 3),
+                     radix_scratch_words(std::max(c.cap_p2p, c.cap_m2l), 1)));
     c.d_front0.ensure(8 * (size_t)c.cap_front);
     c.d_front1.ensure(8 * (size_t)c.cap_front);
     c.d_tcnt.ensure(4 * (size_t)(2 * S.ml + 8));
@@ -1072,6 +1075,17 @@ int fmm4rb_op_p2p(fmm4rb_ctx *ctx, int precision, int64_t N, const double *pos, 
                 S.leaves.push_back((int32_t)k);
             }
         }
+    {
+        // each target's workgroup overwrites its particles' rows: target ranges must
+        // not overlap (e.g. a parent and its child both listed as targets)
+        std::vector<std::pair<int32_t, int32_t>> r;
+        for (int32_t k : S.leaves)
+            if (S.cnt[k] > 0) r.emplace_back(S.lo[k], S.lo[k] + S.cnt[k] - 1);
+        std::sort(r.begin(), r.end());
+        for (size_t i = 1; i < r.size(); ++i)
+            FMM_NEED(r[i].first > r[i - 1].second,
+                     "pair targets have overlapping particle ranges");
+    }
     (void)bboxes;  // cluster boxes are not needed: R is formed from absolute positions
     std::vector<double> sp(8 * (size_t)N);
     std::vector<float> spf(4 * (size_t)N);

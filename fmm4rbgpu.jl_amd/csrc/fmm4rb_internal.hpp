@@ -2,7 +2,7 @@
 //
 // Data layout in HBM (N particles, nc clusters, NP = n+1, NN = NP^3):
 //   sp8   [N][8]  double  particles in tree order: x, y, z, gamma, px, py, pz, 0
-//   sp8f  [N][8]  float   same, positions relative to the particle's leaf centre
+//   sp8f  [N] float4      gamma, px, py, pz (sources of the mixed-precision P2P)
 //   geo   [nc][8] double  centre xyz, admissibility radius, diameter, 0, 0, 0
 //   ncoord[nc][3][NP]     Chebyshev (2nd kind) interpolation nodes per axis
 //   mp4   [nc][NN] double4 multipoles (gamma, mx, my, mz)

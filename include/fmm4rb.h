@@ -201,8 +201,10 @@ int fmm4rb_lists(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mo
  * M2L: gpu_M2L! (src/gpu_fmm/gpu_interaction.jl:3-41); le/lb (3*(n+1)^3*nc)
  *      are overwritten with the M2L sums of the listed pairs.
  * P2P: gpu_P2P! (src/gpu_fmm/gpu_interaction.jl:43-86); E/B (3N) are
- *      overwritten with the P2P sums (no amplitude factor).  bboxes is
- *      accepted for symmetry with the tree hook and may be NULL.
+ *      overwritten with the P2P sums (no amplitude factor).  The targets of
+ *      the pairs must have disjoint particle ranges (else FMM4RB_EINVAL), as
+ *      leaves do.  bboxes is accepted for symmetry with the tree hook and may
+ *      be NULL.
  */
 int fmm4rb_op_m2l(fmm4rb_ctx *ctx, int n, int precision, int64_t nc, const double *bboxes,
                   const double *gam, const double *mmom, int64_t npairs, const int64_t *pairs,

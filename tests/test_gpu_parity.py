@@ -203,6 +203,16 @@ def test_reference_gpu_p2p_far_field(ctx, prec):
     assert isapprox(E[npar - 1], -far, rtol=1e-3) and isapprox(B[npar - 1], [0, 0, 0], rtol=1e-3)
 
 
+def test_op_p2p_rejects_overlapping_targets(ctx):
+    # cluster 0 = particles 0..199 contains cluster 1 = 0..99: both as targets would
+    # let two workgroups overwrite the same rows
+    pos = np.random.default_rng(12).random((200, 3))
+    mom = np.zeros((200, 3))
+    with pytest.raises(F.FMMError) as ex:
+        ctx.op_p2p(pos, mom, np.arange(200), [0, 0, 100], [199, 99, 199], [(0, 2), (1, 2)])
+    assert ex.value.code == 1 and "overlapping" in str(ex.value)
+
+
 def test_op_m2l_matches_oracle(ctx, orc):
     rng = np.random.default_rng(11)
     n, nc = 4, 6
