@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <system_error>
 #include <thread>
@@ -23,8 +24,27 @@ namespace fmm4rb {
 
 static thread_local std::string g_err;
 
+thread_local Arena *t_arena = nullptr;
+
 void DevBuf::ensure(size_t bytes) {
     if (bytes <= cap && p) return;
+    if (t_arena) {
+        // allocation-free call: carve from the arena; the old region (if any) is
+        // the arena's too and is simply left behind
+        Arena &a = *t_arena;
+        const size_t b = std::max<size_t>(bytes, 256);
+        const size_t off = (a.used + 255) & ~(size_t)255;
+        if (off > a.size || b > a.size - off)
+            fail(FMM4RB_ENOMEM, "the call needs more than the " + std::to_string(a.size >> 20) +
+                                    " MiB static scratch of the allocation-free entry point "
+                                    "(at least " + std::to_string((off + b + ((1 << 20) - 1)) >> 20) +
+                                    " MiB so far): use fmm4rb_field_device for this size");
+        p = a.base + off;
+        cap = b;
+        borrowed = true;
+        a.used = off + b;
+        return;
+    }
     release();
     size_t b = std::max<size_t>(bytes, 256);
     hipError_t e = hipMalloc(&p, b);
@@ -39,9 +59,76 @@ void DevBuf::ensure(size_t bytes) {
 }
 
 void DevBuf::release() {
-    if (p) (void)hipFree(p);
+    if (p && !borrowed) (void)hipFree(p);
     p = nullptr;
     cap = 0;
+    borrowed = false;
+}
+
+// ---- static scratch of the allocation-free entry point (fmm4rb_field_device_noalloc)
+// Every device buffer of that entry point's pipeline is carved from this array.
+// In host code its name is only a stand-in for the symbol: its device address is
+// looked up once per device (scratch_init, from fmm4rb_create with the device
+// current) and only that address is used.
+constexpr size_t SCRATCH_BYTES = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+static void scratch_init(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return;  // noalloc then refuses
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        fail(FMM4RB_EDEVICE, std::string("hipGetSymbolAddress of the static scratch failed: ") +
+                                 hipGetErrorString(e));
+    }
+    g_scratch_ptr[device] = ptr;
+}
+
+static void *scratch_ptr(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+// Every device buffer a context owns (destroy releases them; the allocation-free
+// entry point swaps them for arena carves during its call).  Returns the count.
+constexpr int CTX_NBUF_MAX = 80;
+static int ctx_bufs(Ctx &c, DevBuf **out) {
+    DevBuf *all[] = {&c.d_lo, &c.d_cnt, &c.d_flags, &c.d_leaves, &c.d_lvl_nodes, &c.d_lvl_split,
+            &c.d_leftbase, &c.d_part, &c.d_pavg, &c.d_bad, &c.d_keys, &c.d_keys_alt,
+            &c.d_vals, &c.d_vals_alt, &c.d_rs_scratch, &c.d_seg, &c.d_seg_alt,
+            &c.d_rk, &c.d_pivot, &c.d_bsum, &c.d_axis, &c.d_parind, &c.d_leafof,
+            &c.d_bbox, &c.d_geo, &c.d_ncoord, &c.d_costab, &c.d_ext, &c.d_sp8, &c.d_sp8f,
+            &c.d_ob, &c.d_gx, &c.d_gp, &c.d_gx2, &c.d_gp2, &c.d_xbar2, &c.d_lbar, &c.d_mbar,
+            &c.d_xbar, &c.d_pbar, &c.d_abar, &c.d_p2pT, &c.d_p2pT_alt, &c.d_m2lT, &c.d_m2lT_alt,
+            &c.d_p2pT_off, &c.d_m2lT_off, &c.d_front0, &c.d_front1, &c.d_tcnt, &c.d_p2p,
+            &c.d_p2p_alt, &c.d_m2l, &c.d_m2l_alt, &c.d_p2p_off, &c.d_m2l_off, &c.d_mp4,
+            &c.d_mp4f, &c.d_loc, &c.d_near, &c.d_m2lT_pos, &c.d_ysp};
+    constexpr int nb = (int)(sizeof(all) / sizeof(all[0]));
+    static_assert(nb <= CTX_NBUF_MAX, "raise CTX_NBUF_MAX");
+    for (int i = 0; i < nb; ++i) out[i] = all[i];
+    return nb;
+}
+
+// Zero n 32-bit words (a kernel rather than hipMemsetAsync, so that no memset
+// node is recorded if a caller captures the stream).
+__global__ void k_zero_u32(uint32_t *__restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+static void zero_words(void *p, int64_t nwords, hipStream_t st) {
+    if (nwords <= 0) return;
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, st,
+                       (uint32_t *)p, nwords);
+    FMM_LAUNCH_CHECK();
 }
 
 // treeinfo.jl:6-8
@@ -232,7 +319,7 @@ __global__ __launch_bounds__(256) void k_list_stats(const uint64_t *__restrict__
 // p_avg and the input check; returns after the check (one host sync).
 static void stage_pavg(Ctx &c, const double *pos, const double *mom) {
     hipStream_t st = c.stream;
-    FMM_HIP(hipMemsetAsync(c.d_bad.p, 0, sizeof(int), st));
+    zero_words(c.d_bad.p, 1, st);
     launch_pavg(pos, mom, c.N, c.d_part.as<double>(), c.d_pavg.as<double>(), c.d_bad.as<int>(), st);
     int bad = 0;
     FMM_HIP(hipMemcpyAsync(&bad, c.d_bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -321,7 +408,7 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
     if (stats) {
         DevBuf &tmp = c.d_bad;  // reuse: needs 16 bytes
         tmp.ensure(16);
-        FMM_HIP(hipMemsetAsync(tmp.p, 0, 16, st));
+        zero_words(tmp.p, 4, st);
         const int64_t mx = std::max(c.np2p, c.nm2l);
         if (mx > 0) {
             hipLaunchKernelGGL(k_list_stats, dim3((unsigned)std::min<int64_t>(1024, (mx + 255) / 256)),
@@ -580,6 +667,7 @@ int fmm4rb_create(fmm4rb_ctx **out, int device) {
     FMM_HIP(hipGetDeviceCount(&ndev));
     FMM_NEED(device >= 0 && device < ndev, "device index out of range");
     FMM_HIP(hipSetDevice(device));
+    scratch_init(device);  // once per device: the allocation-free entry point's memory
     fmm4rb_ctx *ctx = new fmm4rb_ctx();
     ctx->c.device = device;
     try {
@@ -607,20 +695,12 @@ int fmm4rb_destroy(fmm4rb_ctx *ctx) {
     if (ctx->c.aux) (void)hipStreamSynchronize(ctx->c.aux);
     if (ctx->c.aux2) (void)hipStreamSynchronize(ctx->c.aux2);
     Ctx &c = ctx->c;
-    DevBuf *bufs[] = {&c.d_lo, &c.d_cnt, &c.d_flags, &c.d_leaves, &c.d_lvl_nodes, &c.d_lvl_split,
-                      &c.d_leftbase, &c.d_part, &c.d_pavg, &c.d_bad, &c.d_keys, &c.d_keys_alt,
-                      &c.d_vals, &c.d_vals_alt, &c.d_rs_scratch, &c.d_seg, &c.d_seg_alt,
-                      &c.d_rk, &c.d_pivot, &c.d_bsum, &c.d_axis, &c.d_parind, &c.d_leafof,
-                      &c.d_bbox, &c.d_geo, &c.d_ncoord, &c.d_costab, &c.d_ext, &c.d_sp8, &c.d_sp8f,
-                      &c.d_ob, &c.d_gx, &c.d_gp, &c.d_gx2, &c.d_gp2, &c.d_xbar2, &c.d_lbar, &c.d_mbar,
-                      &c.d_xbar, &c.d_pbar,
-                      &c.d_abar, &c.d_p2pT, &c.d_p2pT_alt, &c.d_m2lT, &c.d_m2lT_alt,
-                      &c.d_p2pT_off, &c.d_m2lT_off,
-                      &c.d_front0, &c.d_front1, &c.d_tcnt, &c.d_p2p, &c.d_p2p_alt, &c.d_m2l,
-                      &c.d_m2l_alt, &c.d_p2p_off, &c.d_m2l_off, &c.d_mp4, &c.d_mp4f, &c.d_loc,
-                      &c.d_near, &c.d_m2lT_pos, &c.d_ysp, &ctx->h_pos, &ctx->h_mom, &ctx->h_E,
-                      &ctx->h_B, &ctx->h_tg, &ctx->h_gpos, &ctx->h_gmom, &ctx->h_ob};
-    for (DevBuf *b : bufs) b->release();
+    DevBuf *bufs[CTX_NBUF_MAX];
+    const int nb = ctx_bufs(c, bufs);
+    for (int i = 0; i < nb; ++i) bufs[i]->release();
+    DevBuf *hbufs[] = {&ctx->h_pos, &ctx->h_mom, &ctx->h_E, &ctx->h_B,
+                       &ctx->h_tg, &ctx->h_gpos, &ctx->h_gmom, &ctx->h_ob};
+    for (DevBuf *b : hbufs) b->release();
     ctx->stage.release();
     for (auto &e : c.ev)
         if (e) (void)hipEventDestroy(e);
@@ -665,6 +745,60 @@ int fmm4rb_field_device(fmm4rb_ctx *ctx, int64_t N, const double *d_pos, const d
     FMM_NEED(ctx && d_pos && d_mom && d_E && d_B, "NULL argument");
     set_device(ctx);
     run_field(ctx->c, N, d_pos, d_mom, charge, lambda, n, N0, eta, precision, d_E, d_B, stats);
+    FMM_API_END
+}
+
+// The allocation-free form of fmm4rb_field_device: the context's own buffers are
+// set aside, every buffer of the call is carved from this device's static scratch
+// (so nothing calls hipMalloc or hipFree), and afterwards the context gets its own
+// buffers back, with its tree, lists and locals marked stale (the scratch is
+// shared by every context on the device).
+int fmm4rb_field_device_noalloc(fmm4rb_ctx *ctx, int64_t N, const double *d_pos,
+                                const double *d_mom, double charge, double lambda, int n,
+                                int64_t N0, double eta, int precision, double *d_E, double *d_B,
+                                fmm4rb_stats *stats) {
+    FMM_API_BEGIN
+    FMM_NEED(ctx && d_pos && d_mom && d_E && d_B, "NULL argument");
+    set_device(ctx);
+    Ctx &c = ctx->c;
+    void *base = scratch_ptr(c.device);
+    if (!base) fail(FMM4RB_EDEVICE, "no static scratch address for this device");
+    validate(N, n, N0, eta, precision);
+    DevBuf *bufs[CTX_NBUF_MAX];
+    DevBuf saved[CTX_NBUF_MAX];
+    const int nb = ctx_bufs(c, bufs);
+    const int64_t caps[3] = {c.cap_front, c.cap_p2p, c.cap_m2l};
+    for (int i = 0; i < nb; ++i) {
+        saved[i] = *bufs[i];
+        *bufs[i] = DevBuf{};
+    }
+    c.cap_front = c.cap_p2p = c.cap_m2l = 0;  // sized from the leaf count by prepare()
+    c.shape.N = -1;                           // the shape arrays are uploaded to the scratch
+    Arena arena;
+    arena.base = static_cast<unsigned char *>(base);
+    arena.size = SCRATCH_BYTES;
+    t_arena = &arena;
+    auto restore = [&] {
+        t_arena = nullptr;
+        for (int i = 0; i < nb; ++i) *bufs[i] = saved[i];
+        c.cap_front = caps[0];
+        c.cap_p2p = caps[1];
+        c.cap_m2l = caps[2];
+        c.shape.N = -1;  // the context's own shape arrays are uploaded again on its next call
+        c.tree_valid = false;
+        c.loc_valid = false;
+    };
+    try {
+        run_field(c, N, d_pos, d_mom, charge, lambda, n, N0, eta, precision, d_E, d_B, stats);
+    } catch (...) {
+        // no work of this call may still use the scratch when the next call starts
+        (void)hipStreamSynchronize(c.stream);
+        (void)hipStreamSynchronize(c.aux);
+        (void)hipStreamSynchronize(c.aux2);
+        restore();
+        throw;
+    }
+    restore();
     FMM_API_END
 }
 

@@ -37,15 +37,27 @@ struct Error {
 
 inline void fail(int code, const std::string &m) { throw Error{code, m}; }
 
-// Grow-only device buffer.
+// Grow-only device buffer.  Inside a call of the allocation-free entry point
+// (an Arena is active on the calling thread) ensure() carves the buffer from
+// that arena instead of calling hipMalloc, and the buffer is "borrowed": it is
+// never passed to hipFree.
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    bool borrowed = false;
     void ensure(size_t bytes);
     void release();
     template <class T>
     T *as() const { return static_cast<T *>(p); }
 };
+
+// Bump allocator over a caller-provided device region (256-byte aligned carves).
+struct Arena {
+    unsigned char *base = nullptr;
+    size_t size = 0, used = 0;
+};
+// The arena of the allocation-free call running on this thread (nullptr outside one).
+extern thread_local Arena *t_arena;
 
 constexpr double EPS64 = 2.220446049250313e-16;  // Julia eps()
 constexpr float EPS32K = 2.220446049250313e-16f;  // the same softening in FP32

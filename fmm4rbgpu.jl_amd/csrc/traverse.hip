@@ -211,8 +211,8 @@ void build_lists_device(Ctx &c, double eta) {
                                c.d_pavg.as<double>(), eta);
             FMM_LAUNCH_CHECK();
         }
-        std::vector<uint32_t> h(ncnt);
-        FMM_HIP(hipMemcpyAsync(h.data(), cnt, sizeof(uint32_t) * ncnt, hipMemcpyDeviceToHost, st));
+        uint32_t h[2 * 29 + 4];  // ncnt <= 2 * maxlevel + 4, maxlevel <= 29
+        FMM_HIP(hipMemcpyAsync(h, cnt, sizeof(uint32_t) * ncnt, hipMemcpyDeviceToHost, st));
         FMM_HIP(hipStreamSynchronize(st));
         uint64_t maxf = 0;
         for (int it = 0; it <= iters; ++it) maxf = std::max<uint64_t>(maxf, h[2 + it]);

@@ -35,7 +35,7 @@ EXPORTS = (
     "fmm4rb_bruteforce", "fmm4rb_tree", "fmm4rb_lists", "fmm4rb_op_m2l", "fmm4rb_op_p2p",
     "fmm4rb_set_debug", "fmm4rb_debug_fetch", "fmm4rb_field_vjp", "fmm4rb_field_vjp_device",
     "fmm4rb_bruteforce_vjp", "fmm4rb_field_refit", "fmm4rb_field_device_refit",
-    "fmm4rb_field_pullback", "fmm4rb_field_pullback_device",
+    "fmm4rb_field_pullback", "fmm4rb_field_pullback_device", "fmm4rb_field_device_noalloc",
 )
 
 
@@ -99,6 +99,7 @@ def lib():
                                    C.c_double, C.c_int, _dp, _dp, C.POINTER(Stats)]
         L.fmm4rb_field_device.argtypes = [_vp, _i64, _vp, _vp, C.c_double, C.c_double, C.c_int,
                                           _i64, C.c_double, C.c_int, _vp, _vp, C.POINTER(Stats)]
+        L.fmm4rb_field_device_noalloc.argtypes = L.fmm4rb_field_device.argtypes
         L.fmm4rb_field_refit.argtypes = L.fmm4rb_field.argtypes
         L.fmm4rb_field_device_refit.argtypes = L.fmm4rb_field_device.argtypes
         L.fmm4rb_bruteforce.argtypes = [_vp, _i64, _dp, _dp, C.c_double, C.c_double, _i64, _vp,
@@ -201,6 +202,16 @@ class Context:
         st = Stats()
         _check(lib().fmm4rb_field_device(self._h, N, pos_ptr, mom_ptr, charge, lam, n, N0, eta,
                                          _PREC[precision], E_ptr, B_ptr, C.byref(st)))
+        return st
+
+    def field_device_noalloc(self, N, pos_ptr, mom_ptr, E_ptr, B_ptr, charge=-1.0, lam=1.0,
+                             n=4, N0=125, eta=0.5, precision="fp64"):
+        """field_device() without device allocation: every buffer of the call comes
+        from the library's static per-device scratch (N up to about 2^18 at n=4)."""
+        st = Stats()
+        _check(lib().fmm4rb_field_device_noalloc(self._h, N, pos_ptr, mom_ptr, charge, lam, n,
+                                                 N0, eta, _PREC[precision], E_ptr, B_ptr,
+                                                 C.byref(st)))
         return st
 
     def field_refit(self, pos, mom, charge=-1.0, lam=1.0, n=4, N0=125, eta=0.5,

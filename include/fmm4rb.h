@@ -90,6 +90,28 @@ int fmm4rb_field_device(fmm4rb_ctx *ctx, int64_t N, const double *d_pos, const d
                         int precision, double *d_E, double *d_B, fmm4rb_stats *stats);
 
 /*
+ * Allocation-free variant of fmm4rb_field_device.  No reference counterpart: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, semantics and results as fmm4rb_field_device (bit-identical), but
+ * every device buffer of the call is carved from a static scratch array of
+ * 256 MiB per device (each device the library runs on holds it until the
+ * process ends); nothing calls hipMalloc, hipFree or another allocating
+ * routine.  A call whose buffers exceed the scratch fails with FMM4RB_ENOMEM
+ * before any kernel runs (about N <= 2^18 at n = 4, fewer at larger n); if the
+ * interaction lists outgrow their capacity estimate (64 pairs per leaf) it can
+ * fail the same way after the traversal.  Either way E and B are not written.
+ * The scratch is shared by every context and thread on a device and keeps the
+ * last call's contents: calls that overlap on one device race.  The context's
+ * own buffers are untouched; its tree, lists and locals count as stale
+ * afterwards (a following refit or pullback fails with FMM4RB_EINVAL).
+ */
+int fmm4rb_field_device_noalloc(fmm4rb_ctx *ctx, int64_t N, const double *d_pos,
+                                const double *d_mom, double charge, double lambda, int n,
+                                int64_t N0, double eta, int precision, double *d_E, double *d_B,
+                                fmm4rb_stats *stats);
+
+/*
  * Tracking steps (no reference counterpart: the reference rebuilds the tree on
  * every call, update_particles_field.jl:96-104).  The field for new positions and
  * momenta on the tree and interaction lists of this context's last completed
