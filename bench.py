@@ -42,6 +42,9 @@ def parse():
                     help="particles of the serial CPU FMM sample (0 = skip)")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--large-N", type=int, default=25_600_000,
+                    help="particles of the large uniform-ellipsoid case reported beside the "
+                         "headline (FP64 and mixed forward, one FP64 reverse pass; 0 = skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -99,17 +102,49 @@ def roofline(st, stages, n, prec, N):
                       "frac": of / (oms * 1e-3) / PEAK[prec], "avg_launch_ms": oms}}
 
 
+def host_cpu():
+    """CPU model and the cores this process may use (the host's share)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(avail or 1, int(omp)) if omp.isdigit() and int(omp) > 0 else (avail or 1)
+    return {"cpu_model": model, "nproc_available": avail, "os_cpu_count": os.cpu_count(),
+            "OMP_NUM_THREADS": omp or None, "threads_used": threads}
+
+
 def cpu_baseline(N, n, N0, eta, N_full):
     import numpy as np
 
     import oracle
     from fmm4rb_hip import workloads as W
+    host = host_cpu()
+    threads = host["threads_used"]
     pos, mom = W.bunch(N, seed=2)
+    oracle.set_threads(1)
     t0 = time.perf_counter()
     oracle.fmm(pos, mom, -1.0, 1e-3, n, N0, eta)
     dt = time.perf_counter() - t0
+    # the N = 8192 unit-cube case on 1 thread, and the N = 2^20 bunch on every core
+    # (the oracle's loops over target groups on OpenMP threads, bit-identical to serial)
+    pc, mc = W.uniform_cube(8192, seed=1)
+    t0 = time.perf_counter()
+    oracle.fmm(pc, mc, -1.0, 1.0, n, N0, eta)
+    dt1 = time.perf_counter() - t0
+    p2, m2 = W.bunch(1 << 20, seed=2)
+    oracle.set_threads(threads)
+    t0 = time.perf_counter()
+    oracle.fmm(p2, m2, -1.0, 1e-3, n, N0, eta)
+    dt2 = time.perf_counter() - t0
+    oracle.set_threads(0)
     # BruteForce (the accuracy oracle) on the host's cores: all N sources, 4096 targets
-    threads = max(1, min(16, os.cpu_count() or 1))
     tg = np.random.default_rng(7).choice(N, 4096, replace=False)
     t0 = time.perf_counter()
     oracle.bruteforce(pos, mom, -1.0, 1e-3, targets=tg, nthreads=threads)
@@ -118,10 +153,66 @@ def cpu_baseline(N, n, N0, eta, N_full):
     return {"value": N / dt, "unit": "particles/s", "cores": 1, "kind": "port",
             "sample": f"serial FMM (C restatement of src/fmm, 1 thread) on a {N}-particle bunch "
                       f"of the same distribution, n={n} N0={N0} eta={eta}: {dt:.1f} s",
+            "host": host,
+            "fmm_uniform8192_1thread_s": dt1,
+            "fmm_bunch_2e20_all_threads": {"s": dt2, "particles_per_s": (1 << 20) / dt2,
+                                           "cores": threads},
             "bruteforce": {"pair_evals_per_s": pairs_s, "cores": threads, "kind": "port",
                            "sample": f"BruteForce (C restatement of update_particles_field.jl:21-38, "
                                      f"OpenMP) on the same {N} sources x 4096 targets: {dtb:.2f} s",
                            "est_s_full_workload": float(N_full) * float(N_full) / pairs_s}}
+
+
+def large_case(F, W, torch, N, n, N0, eta, lam):
+    """The N = 25.6M uniform ellipsoid (semi-axes 1e-3, 1e-3, 1e-4; p_z ~ N(200, 2)),
+    the reference README's problem size: FP64 and mixed forward (1 warm-up, 2 timed
+    calls each), one FP64 reverse pass, mixed vs FP64 and FP64 vs GPU BruteForce on
+    1024 targets.  Its own context, closed afterwards."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    pos, mom = W.ellipsoid(N, seed=5)
+    dpos, dmom = torch.from_numpy(pos).to(dev), torch.from_numpy(mom).to(dev)
+    out = {"workload": f"uniform ellipsoid, N={N}, semi-axes (1e-3,1e-3,1e-4), p_z~N(200,2), "
+                       "p_x,p_y~N(0,0.02), lambda=1e-3", "N": N, "n": n, "N0": N0, "eta": eta}
+    fields = {}
+    with F.Context(0) as ctx:
+        for prec in ("fp64", "mixed"):
+            E, B = torch.empty_like(dpos), torch.empty_like(dpos)
+            call = lambda: ctx.field_device(N, dpos.data_ptr(), dmom.data_ptr(), E.data_ptr(),
+                                            B.data_ptr(), -1.0, lam, n, N0, eta, prec)
+            call()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = [call(), call()]
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / 2 * 1e3
+            out[prec + "_forward_ms"] = ms
+            out[prec + "_particles_per_s"] = N / (ms * 1e-3)
+            out[prec + "_stages_ms"] = stage_avg(st)
+            fields[prec] = (E, B)
+        E, B = fields["fp64"]
+        w = torch.rand(N, 1, dtype=torch.float64, device=dev,
+                       generator=torch.Generator(device=dev).manual_seed(43))
+        Eb, Bb = 2 * w * E, 2 * w * B
+        gpos, gmom = torch.empty_like(dpos), torch.empty_like(dpos)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.field_vjp_device(N, dpos.data_ptr(), dmom.data_ptr(), Eb.data_ptr(), Bb.data_ptr(),
+                             gpos.data_ptr(), gmom.data_ptr(), -1.0, lam, n, N0, eta, "fp64")
+        torch.cuda.synchronize()
+        out["fp64_reverse_ms"] = (time.perf_counter() - t0) * 1e3
+        out["fp64_reverse_ratio_to_forward"] = out["fp64_reverse_ms"] / out["fp64_forward_ms"]
+        del Eb, Bb, gpos, gmom, w
+        Ed, Bd = E.cpu().numpy(), B.cpu().numpy()
+        Em, Bm = fields["mixed"][0].cpu().numpy(), fields["mixed"][1].cpu().numpy()
+        tg = np.random.default_rng(42).choice(N, 1024, replace=False)
+        Ebf, Bbf = ctx.bruteforce(pos, mom, -1.0, lam, targets=tg)
+    out["mixed_vs_fp64"] = {"E": W.rel_l2(Em, Ed), "B": W.rel_l2(Bm, Bd)}
+    out["fp64_vs_bruteforce_1024"] = {"E": W.rel_l2(Ed[tg], Ebf), "B": W.rel_l2(Bd[tg], Bbf)}
+    out["speedup_mixed_vs_fp64"] = out["fp64_forward_ms"] / out["mixed_forward_ms"]
+    del fields, dpos, dmom
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -197,8 +288,10 @@ def main():
             sa = [call() for _ in range(a.steps)]
             torch.cuda.synchronize()
             ms_a = (time.perf_counter() - t0) / a.steps
-            res["adjoint_" + prec] = {"ms_per_step": ms_a * 1e3, "ratio_to_fp64_forward": ms_a / ms_d,
-                                      "stages_ms": stage_avg(sa)}
+            fwd = ms_d if prec == "fp64" else ms_m
+            res["adjoint_" + prec] = {"ms_per_step": ms_a * 1e3,
+                                      "ratio_to_forward": ms_a / fwd,  # same precision
+                                      "forward_ms": fwd * 1e3, "stages_ms": stage_avg(sa)}
             grads[prec] = (gpos.cpu().numpy(), gmom.cpu().numpy())
         res["adjoint_mixed"]["vs_fp64"] = {
             "gpos": W.rel_l2(grads["mixed"][0], grads["fp64"][0]),
@@ -222,8 +315,10 @@ def main():
             torch.cuda.synchronize()
             ms_p = (time.perf_counter() - t0) / a.steps
             g = (gpos.cpu().numpy(), gmom.cpu().numpy())
+            fwd = ms_d if prec == "fp64" else ms_m
             res["pullback_" + prec] = {
-                "ms_per_step": ms_p * 1e3, "ratio_to_fp64_forward": ms_p / ms_d,
+                "ms_per_step": ms_p * 1e3, "ratio_to_forward": ms_p / fwd,  # same precision
+                "forward_ms": fwd * 1e3,
                 "stages_ms": stage_avg(sp),
                 "vs_vjp": {"gpos": W.rel_l2(g[0], grads[prec][0]),
                            "gmom": W.rel_l2(g[1], grads[prec][1])}}
@@ -274,6 +369,8 @@ def main():
         out["fp64"] = res["fp64"]
         out["speedup_vs_fp64"] = out["value"] / res["fp64"]["value"]
     ctx.close()
+    if a.large_N > 0:
+        out["large_ellipsoid"] = large_case(F, W, torch, a.large_N, a.n, a.N0, a.eta, lam)
     if a.copy_probe_mib > 0:
         src = torch.ones(a.copy_probe_mib << 18, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)

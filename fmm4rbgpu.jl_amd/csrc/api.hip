@@ -631,7 +631,7 @@ struct HostStage {
 
 struct fmm4rb_ctx {
     Ctx c;
-    DevBuf h_pos, h_mom, h_E, h_B, h_tg, h_gpos, h_gmom, h_ob;
+    DevBuf h_pos, h_mom, h_E, h_B, h_tg, h_gpos, h_gmom, h_ob, h_bfpart;
     HostStage stage;
 };
 
@@ -699,7 +699,7 @@ int fmm4rb_destroy(fmm4rb_ctx *ctx) {
     const int nb = ctx_bufs(c, bufs);
     for (int i = 0; i < nb; ++i) bufs[i]->release();
     DevBuf *hbufs[] = {&ctx->h_pos, &ctx->h_mom, &ctx->h_E, &ctx->h_B,
-                       &ctx->h_tg, &ctx->h_gpos, &ctx->h_gmom, &ctx->h_ob};
+                       &ctx->h_tg, &ctx->h_gpos, &ctx->h_gmom, &ctx->h_ob, &ctx->h_bfpart};
     for (DevBuf *b : hbufs) b->release();
     ctx->stage.release();
     for (auto &e : c.ev)
@@ -987,7 +987,7 @@ int fmm4rb_bruteforce(fmm4rb_ctx *ctx, int64_t N, const double *pos, const doubl
     if (nt > 0) {
         const double amp = (AMP_RE * charge) / lambda;
         launch_bruteforce(ctx->h_pos.as<double>(), ctx->h_mom.as<double>(), N, dt, nt, amp,
-                          ctx->h_E.as<double>(), ctx->h_B.as<double>(), st);
+                          ctx->h_E.as<double>(), ctx->h_B.as<double>(), ctx->h_bfpart, st);
         FMM_HIP(hipMemcpyAsync(E, ctx->h_E.p, obytes, hipMemcpyDeviceToHost, st));
         FMM_HIP(hipMemcpyAsync(B, ctx->h_B.p, obytes, hipMemcpyDeviceToHost, st));
     }

@@ -4,7 +4,14 @@
 // (src/update_particles_field.jl:21-38), the accuracy oracle of the FMM.
 // Same expression order as the reference: K = R / sqrt(R.R + (p.R)^2 + eps)^3,
 // E += (amp * gamma_j) * K, B += amp * (p_j x K), sources in index order.
-// One thread per target; sources are staged through LDS in tiles.
+// One thread per target; sources are staged through LDS in tiles.  With few
+// targets (e.g. 4096 sampled targets of a large bunch) the source range is split
+// into C chunks, one workgroup per (target block, chunk), so that the launch
+// fills the GPU; the C partial sums of each target are added by a second kernel
+// in chunk order (deterministic; the order of additions differs from the single
+// running sum only by that final grouping).
+#include <algorithm>
+
 #include "fmm4rb_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -13,21 +20,26 @@ namespace fmm4rb {
 
 constexpr int BF_T = 256;
 
+// Block (x, c): targets x * BF_T .. + BF_T over sources [c * chunk, (c + 1) * chunk).
+// One chunk (gridDim.y == 1): the sums go to E, B; else to part[c][a][6].
 __global__ __launch_bounds__(BF_T) void k_bruteforce(const double *__restrict__ pos,
                                                      const double *__restrict__ mom, int64_t N,
                                                      const int64_t *__restrict__ targets, int64_t nt,
-                                                     double amp, double *__restrict__ E,
-                                                     double *__restrict__ B) {
+                                                     double amp, int64_t chunk,
+                                                     double *__restrict__ E,
+                                                     double *__restrict__ B,
+                                                     double *__restrict__ part) {
     __shared__ double tile[BF_T][8];
     const int64_t a = (int64_t)blockIdx.x * BF_T + threadIdx.x;
     const bool act = a < nt;
     const int64_t i = act ? (targets ? targets[a] : a) : 0;
     const double xi0 = pos[3 * i], xi1 = pos[3 * i + 1], xi2 = pos[3 * i + 2];
     double e[3] = {0.0, 0.0, 0.0}, b[3] = {0.0, 0.0, 0.0};
-    for (int64_t j0 = 0; j0 < N; j0 += BF_T) {
+    const int64_t jb = (int64_t)blockIdx.y * chunk, je = min(N, jb + chunk);
+    for (int64_t j0 = jb; j0 < je; j0 += BF_T) {
         __syncthreads();
         const int64_t j = j0 + threadIdx.x;
-        if (j < N) {
+        if (j < je) {
             const double m0 = mom[3 * j], m1 = mom[3 * j + 1], m2 = mom[3 * j + 2];
             tile[threadIdx.x][0] = pos[3 * j];
             tile[threadIdx.x][1] = pos[3 * j + 1];
@@ -38,7 +50,7 @@ __global__ __launch_bounds__(BF_T) void k_bruteforce(const double *__restrict__ 
             tile[threadIdx.x][6] = m2;
         }
         __syncthreads();
-        const int m = (int)min<int64_t>(BF_T, N - j0);
+        const int m = (int)min<int64_t>(BF_T, je - j0);
         if (act) {
             for (int jj = 0; jj < m; ++jj) {
                 const double *t = tile[jj];
@@ -58,11 +70,35 @@ __global__ __launch_bounds__(BF_T) void k_bruteforce(const double *__restrict__ 
             }
         }
     }
-    if (act)
+    if (!act) return;
+    if (gridDim.y == 1) {
         for (int d = 0; d < 3; ++d) {
             E[3 * a + d] = e[d];
             B[3 * a + d] = b[d];
         }
+    } else {
+        double *o = part + ((int64_t)blockIdx.y * nt + a) * 6;
+        for (int d = 0; d < 3; ++d) {
+            o[d] = e[d];
+            o[3 + d] = b[d];
+        }
+    }
+}
+
+// E, B of target a = the chunks' partial sums added in chunk order.
+__global__ __launch_bounds__(BF_T) void k_bf_reduce(const double *__restrict__ part, int nch,
+                                                    int64_t nt, double *__restrict__ E,
+                                                    double *__restrict__ B) {
+    const int64_t a = (int64_t)blockIdx.x * BF_T + threadIdx.x;
+    if (a >= nt) return;
+    double v[6];
+    for (int d = 0; d < 6; ++d) v[d] = part[a * 6 + d];
+    for (int c = 1; c < nch; ++c)
+        for (int d = 0; d < 6; ++d) v[d] += part[((int64_t)c * nt + a) * 6 + d];
+    for (int d = 0; d < 3; ++d) {
+        E[3 * a + d] = v[d];
+        B[3 * a + d] = v[3 + d];
+    }
 }
 
 // Reverse mode of BruteForce: for every ordered pair (i, j), u = g_j e_i + b_i x p_j,
@@ -192,10 +228,26 @@ void launch_bruteforce_vjp(const double *pos, const double *mom, int64_t N, cons
 }
 
 void launch_bruteforce(const double *pos, const double *mom, int64_t N, const int64_t *targets,
-                       int64_t nt, double amp, double *E, double *B, hipStream_t st) {
-    hipLaunchKernelGGL(k_bruteforce, dim3((unsigned)((nt + BF_T - 1) / BF_T)), dim3(BF_T), 0, st,
-                       pos, mom, N, targets, nt, amp, E, B);
+                       int64_t nt, double amp, double *E, double *B, DevBuf &part,
+                       hipStream_t st) {
+    const int64_t tb = (nt + BF_T - 1) / BF_T;
+    // chunks of whole tiles: about 2048 workgroups in all (8 per CU), at most 1024
+    // chunks, each at least one tile
+    const int64_t tiles = (N + BF_T - 1) / BF_T;
+    const int64_t want = std::min<int64_t>(1024, std::max<int64_t>(1, 2048 / tb));
+    const int64_t tpc = (tiles + want - 1) / want;  // tiles per chunk
+    const int64_t chunk = tpc * BF_T;
+    const int nch = (int)((N + chunk - 1) / chunk);
+    if (nch > 1) part.ensure((size_t)nch * (size_t)nt * 6 * sizeof(double));
+    hipLaunchKernelGGL(k_bruteforce, dim3((unsigned)tb, (unsigned)nch), dim3(BF_T), 0, st, pos,
+                       mom, N, targets, nt, amp, chunk, E, B,
+                       nch > 1 ? part.as<double>() : (double *)nullptr);
     FMM_LAUNCH_CHECK();
+    if (nch > 1) {
+        hipLaunchKernelGGL(k_bf_reduce, dim3((unsigned)tb), dim3(BF_T), 0, st, part.as<double>(),
+                           nch, nt, E, B);
+        FMM_LAUNCH_CHECK();
+    }
 }
 
 }  // namespace fmm4rb

@@ -154,8 +154,10 @@ void launch_bruteforce_vjp(const double *pos, const double *mom, int64_t N, cons
                            double *gpos, double *gmom, hipStream_t st);
 
 // ---- brute force (bruteforce.hip)
+// part: scratch for the per-chunk partial sums (grown as needed)
 void launch_bruteforce(const double *pos, const double *mom, int64_t N, const int64_t *targets,
-                       int64_t nt, double amp, double *E, double *B, hipStream_t st);
+                       int64_t nt, double amp, double *E, double *B, DevBuf &part,
+                       hipStream_t st);
 
 struct Ctx {
     int device = 0;

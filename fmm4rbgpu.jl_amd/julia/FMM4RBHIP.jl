@@ -2,10 +2,11 @@
 #
 # Adds the algorithm type `FMMHIP` with the reference's (n, N0, eta) fields
 # (src/update_particles_field.jl:9-19) plus a precision, and a method of
-# `update_particles_field!` for it.  When FMM4RBGPU is loaded the method extends
-# FMM4RBGPU.update_particles_field!; otherwise the module defines a standalone
-# `Particles`-compatible entry point `update_fields!`.  `fields` is the
-# non-mutating form; when ChainRulesCore is loaded it gets an `rrule` whose
+# `update_particles_field!` for it.  Whenever FMM4RBGPU is loaded -- before or after
+# this module, into Main or into any other module -- the method extends
+# FMM4RBGPU.update_particles_field! (found by the package's UUID); the
+# `Particles`-free entry point `update_fields!` works without it.  `fields` is the
+# non-mutating form; whenever ChainRulesCore is loaded it gets an `rrule` whose
 # pullback calls the hand-written adjoint (fmm4rb_field_vjp).
 #
 # Julia is not installed where this library is built and tested; the C ABI is
@@ -145,33 +146,76 @@ function field_vjp(positions::AbstractVector, momenta::AbstractVector, Ebar::Abs
     return gpos, gmom
 end
 
-# Extend the reference's generic function when the reference package is present.
+# The reference package and ChainRulesCore, identified by UUID (FMM4RBGPU's
+# Project.toml; ChainRulesCore's registry entry), wherever they are loaded from.
+const FMM4RBGPU_ID = Base.PkgId(Base.UUID("6188f75b-cc19-42af-84de-782f8f05992b"), "FMM4RBGPU")
+const CHAINRULES_ID = Base.PkgId(Base.UUID("d360d2e6-b24c-11e9-a2a3-2a2ae2dbcce4"),
+                                 "ChainRulesCore")
+const EXTENDED = Ref(false)
+const RRULE_DEFINED = Ref(false)
+
+# update_particles_field!(::FMM4RBGPU.Particles{Float64}, ::FMMHIP; lambda), the drop-in
+# for the FMMGPU method (src/update_particles_field.jl:66-148).
+function extend_reference!(FM::Module)
+    EXTENDED[] && return nothing
+    @eval function $(FM).update_particles_field!(particles::$(FM).Particles{Float64},
+                                                 alg::FMMHIP; lambda)
+        update_fields!(particles.positions, particles.momenta, particles.efields,
+                       particles.bfields, particles.charge, alg; lambda = lambda)
+    end
+    EXTENDED[] = true
+    return nothing
+end
+
+# ChainRulesCore rule for `fields` (charge, alg and lambda are parameters).
+function define_rrule!(CR::Module)
+    RRULE_DEFINED[] && return nothing
+    @eval function $(CR).rrule(::typeof(fields), positions, momenta, charge, alg::FMMHIP;
+                               lambda)
+        E, B = fields(positions, momenta, charge, alg; lambda = lambda)
+        function fields_pullback(ybar)
+            Eb, Bb = $(CR).unthunk(ybar)
+            Eb = Eb isa $(CR).AbstractZero ? zero(E) : collect(Eb)
+            Bb = Bb isa $(CR).AbstractZero ? zero(B) : collect(Bb)
+            gpos, gmom = field_vjp(positions, momenta, Eb, Bb, charge, alg; lambda = lambda,
+                                   after_forward = true)
+            return ($(CR).NoTangent(), gpos, gmom, $(CR).NoTangent(), $(CR).NoTangent())
+        end
+        return (E, B), fields_pullback
+    end
+    RRULE_DEFINED[] = true
+    return nothing
+end
+
+function on_package_load(id::Base.PkgId)
+    id == FMM4RBGPU_ID && extend_reference!(Base.loaded_modules[id])
+    id == CHAINRULES_ID && define_rrule!(Base.loaded_modules[id])
+    return nothing
+end
+
+"""
+    update_particles_field!(particles, alg::FMMHIP; lambda)
+
+The reference API through this module's name: forwards to
+`FMM4RBGPU.update_particles_field!` (which has the `FMMHIP` method once FMM4RBGPU is
+loaded) and fails with an explanation when FMM4RBGPU is not loaded.
+"""
+function update_particles_field!(particles, alg::FMMHIP; lambda)
+    haskey(Base.loaded_modules, FMM4RBGPU_ID) ||
+        error("FMM4RBHIP: update_particles_field! needs the FMM4RBGPU package (its " *
+              "Particles type); load it with `using FMM4RBGPU`, or call " *
+              "update_fields!(positions, momenta, efields, bfields, charge, alg; lambda)")
+    FM = Base.loaded_modules[FMM4RBGPU_ID]
+    extend_reference!(FM)
+    return Base.invokelatest(FM.update_particles_field!, particles, alg; lambda = lambda)
+end
+
 function __init__()
-    if isdefined(Main, :FMM4RBGPU)
-        FM = getfield(Main, :FMM4RBGPU)
-        @eval function $(FM).update_particles_field!(particles::$(FM).Particles{Float64},
-                                                     alg::FMMHIP; lambda)
-            update_fields!(particles.positions, particles.momenta, particles.efields,
-                           particles.bfields, particles.charge, alg; lambda = lambda)
-        end
+    # packages loaded before this module, then every package loaded after it
+    for id in (FMM4RBGPU_ID, CHAINRULES_ID)
+        haskey(Base.loaded_modules, id) && on_package_load(id)
     end
-    # ChainRulesCore rule for `fields` (charge, alg and lambda are parameters).
-    if isdefined(Main, :ChainRulesCore)
-        CR = getfield(Main, :ChainRulesCore)
-        @eval function $(CR).rrule(::typeof(fields), positions, momenta, charge, alg::FMMHIP;
-                                   lambda)
-            E, B = fields(positions, momenta, charge, alg; lambda = lambda)
-            function fields_pullback(ybar)
-                Eb, Bb = $(CR).unthunk(ybar)
-                Eb = Eb isa $(CR).AbstractZero ? zero(E) : collect(Eb)
-                Bb = Bb isa $(CR).AbstractZero ? zero(B) : collect(Bb)
-                gpos, gmom = field_vjp(positions, momenta, Eb, Bb, charge, alg; lambda = lambda,
-                                       after_forward = true)
-                return ($(CR).NoTangent(), gpos, gmom, $(CR).NoTangent(), $(CR).NoTangent())
-            end
-            return (E, B), fields_pullback
-        end
-    end
+    push!(Base.package_callbacks, on_package_load)
 end
 
 end # module
