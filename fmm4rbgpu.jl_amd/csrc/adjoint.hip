@@ -1082,18 +1082,19 @@ __device__ inline T pair_geo(double R0d, double R1d, double R2d, T px, T py, T p
 // Target side: gx_i += sum_j J^T u_ij,  u_ij = g_j e_i + b_i x p_j.
 // One staged tile for the lane's PNG targets; DIAG: the tile holds the targets'
 // own leaf, and the self pair (source k == selfk + 16 q) is skipped.
-template <bool DIAG, typename T, typename V4>
+// NT targets per lane, 64 / NT lanes per group.
+template <bool DIAG, int NT, typename T, typename V4>
 __device__ __forceinline__ void adj_t_tile(const double4 *sx, const V4 *sq, int g, int m,
-                                           int selfk, const double (&xi)[PNG][3],
-                                           const T (&e)[PNG][3], const T (&b)[PNG][3],
-                                           T (&part)[PNG][3]) {
-    for (int k = g; k < m; k += PNG) {
+                                           int selfk, const double (&xi)[NT][3],
+                                           const T (&e)[NT][3], const T (&b)[NT][3],
+                                           T (&part)[NT][3]) {
+    for (int k = g; k < m; k += NT) {
         const double4 xs = sx[k];
         const V4 sqk = sq[k];  // gamma, px, py, pz
 #pragma unroll
-        for (int q = 0; q < PNG; ++q) {
+        for (int q = 0; q < NT; ++q) {
             T R[3], pr;
-            const bool self = DIAG && k == selfk + 16 * q;
+            const bool self = DIAG && k == selfk + (64 / NT) * q;
             const T y = pair_geo<T>(xi[q][0] - xs.x, xi[q][1] - xs.y, xi[q][2] - xs.z, sqk.y,
                                     sqk.z, sqk.w, self, R, pr);
             const T y2 = y * y, y3 = y2 * y;
@@ -1117,25 +1118,29 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ le
                                                   const int32_t *__restrict__ cnt,
                                                   const double *__restrict__ sp8,
                                                   const double *__restrict__ ob,
-                                                  double *__restrict__ gx) {
+                                                  double *__restrict__ gx, int TC) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using V4 = typename std::conditional<MIXED, float4, double4>::type;
-    extern __shared__ double4 adj_lds[];  // sized by the launch: blockDim.x entries each
+    // targets per lane: 4 (16 lanes x 4 groups).  FP64 with 2 (32 lanes x 2 groups,
+    // 118 instead of 220 VGPRs, 4 waves per SIMD instead of 2) ran slower, 22.8 ->
+    // 24.0 ms at N = 2^22: twice the LDS reads per pair.
+    constexpr int NT = PNG, TL = 64 / NT;
+    extern __shared__ double4 adj_lds[];  // sized by the launch: TC entries each
     const int BT = blockDim.x, tid = threadIdx.x;
     double4 *sx = adj_lds;
-    V4 *sq = reinterpret_cast<V4 *>(adj_lds + BT);
-    const int lane = tid & 63, li = lane & 15, g = lane >> 4, wbase = tid - lane;
+    V4 *sq = reinterpret_cast<V4 *>(adj_lds + TC);
+    const int lane = tid & 63, li = lane % TL, g = lane / TL, wbase = tid - lane;
     const int32_t t = leaves[xcd_item(blockIdx.x, gridDim.x)];
     const int32_t a = lo[t], ct = cnt[t];
     const int32_t beg = off[t], end = off[t + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
     const double4 *p8 = reinterpret_cast<const double4 *>(sp8);
     for (int tb = 0; tb < ct; tb += BT) {
-        double xi[PNG][3], acc[PNG][3];
-        T e[PNG][3], b[PNG][3];
+        double xi[NT][3], acc[NT][3];
+        T e[NT][3], b[NT][3];
 #pragma unroll
-        for (int q = 0; q < PNG; ++q) {
-            const int64_t ii = a + min(tb + wbase + li + 16 * q, ct - 1);
+        for (int q = 0; q < NT; ++q) {
+            const int64_t ii = a + min(tb + wbase + li + TL * q, ct - 1);
             const double4 x4 = p8[2 * ii];
             xi[q][0] = x4.x;
             xi[q][1] = x4.y;
@@ -1147,45 +1152,71 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ le
                 acc[q][d] = 0.0;
             }
         }
-        for (int32_t ei = beg; ei < end; ++ei) {
-            const int32_t s = (int32_t)(keys[ei] & mask);
-            const int32_t sa = lo[s], cs = cnt[s];
-            for (int s0 = 0; s0 < cs; s0 += BT) {
-                __syncthreads();
-                const int j = s0 + tid;
-                if (j < cs) {
-                    const double4 x4 = p8[2 * (int64_t)(sa + j)];
-                    const double4 p4 = p8[2 * (int64_t)(sa + j) + 1];
-                    sx[tid] = x4;
-                    sq[tid] = V4{(T)x4.w, (T)p4.x, (T)p4.y, (T)p4.z};
-                }
-                __syncthreads();
-                const int m = min(BT, cs - s0);
-                T part[PNG][3];
-#pragma unroll
-                for (int q = 0; q < PNG; ++q) part[q][0] = part[q][1] = part[q][2] = 0;
-                if (t == s)
-                    adj_t_tile<true>(sx, sq, g, m, tb + wbase + li - s0, xi, e, b, part);
-                else
-                    adj_t_tile<false>(sx, sq, g, m, 0, xi, e, b, part);
-#pragma unroll
-                for (int q = 0; q < PNG; ++q)
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) acc[q][d] += (double)part[q][d];
+        // Sources staged in tiles of up to TC entries, consecutive leaves packed
+        // together; the target leaf itself gets tiles of its own (the self pair).
+        int fill = 0;
+        auto stage = [&](int64_t src0, int slot0, int take) {
+#pragma unroll 1
+            for (int j = tid; j < take; j += BT) {
+                const double4 x4 = p8[2 * (src0 + j)];
+                const double4 p4 = p8[2 * (src0 + j) + 1];
+                sx[slot0 + j] = x4;
+                sq[slot0 + j] = V4{(T)x4.w, (T)p4.x, (T)p4.y, (T)p4.z};
             }
+        };
+        // each tile: staged, then processed at this one call site
+        int ei = beg, p0 = 0;  // next source leaf of the list, its first unstaged particle
+        for (;;) {
+            bool self = false;
+            int selfk = 0;
+            fill = 0;
+            while (ei < end && fill < TC) {
+                const int32_t s = (int32_t)(keys[ei] & mask);
+                const int32_t sa = lo[s], cs = cnt[s];
+                if (s == t && fill > 0) break;  // the target leaf starts a tile of its own
+                const int take = s == t ? min(TC, cs - p0) : min(cs - p0, TC - fill);
+                stage(sa + p0, fill, take);
+                if (s == t) {
+                    self = true;
+                    selfk = tb + wbase + li - p0;
+                }
+                fill += take;
+                p0 += take;
+                if (p0 == cs) {
+                    ++ei;
+                    p0 = 0;
+                }
+                if (self) break;
+            }
+            if (fill == 0) break;
+            __syncthreads();
+            T part[NT][3];
+#pragma unroll
+            for (int q = 0; q < NT; ++q) part[q][0] = part[q][1] = part[q][2] = 0;
+            if (self) adj_t_tile<true, NT>(sx, sq, g, fill, selfk, xi, e, b, part);
+            else adj_t_tile<false, NT>(sx, sq, g, fill, 0, xi, e, b, part);
+#pragma unroll
+            for (int q = 0; q < NT; ++q)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) acc[q][d] += (double)part[q][d];
+            __syncthreads();
         }
 #pragma unroll
-        for (int q = 0; q < PNG; ++q)
+        for (int q = 0; q < NT; ++q)
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                acc[q][d] += __shfl_xor(acc[q][d], 16);
+                if (NT >= 4) acc[q][d] += __shfl_xor(acc[q][d], 16);
                 acc[q][d] += __shfl_xor(acc[q][d], 32);
             }
         double o[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-            o[d] = g == 0 ? acc[0][d] : g == 1 ? acc[1][d] : g == 2 ? acc[2][d] : acc[3][d];
-        const int i = tb + wbase + li + 16 * g;
+        for (int d = 0; d < 3; ++d) {
+            o[d] = acc[0][d];
+#pragma unroll
+            for (int q = 1; q < NT; ++q)
+                if (g == q) o[d] = acc[q][d];
+        }
+        const int i = tb + wbase + li + TL * g;
         if (i < ct) {
             const int64_t ii = a + i;
             for (int d = 0; d < 3; ++d) gx[3 * ii + d] += o[d];
@@ -1212,8 +1243,11 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
     // sums: E3 [0-2], B3 [3-5], sum c R [6-8], sum c pr [9], sum y3 e.R [10],
     //       sum y3 R x b [11-13], sum c pr R [14-16]
     constexpr int NA = 17;
-    // FP64 keeps 17 FP64 sums per source: 2 sources per lane (32 lanes x 2 groups)
-    constexpr int NG = MIXED ? PNG : 2, TL = 64 / NG;
+    // 2 sources per lane (32 lanes x 2 groups): FP64 keeps 17 FP64 sums per source;
+    // MIXED 17 FP32 tile partials and 17 FP64 sums (4 sources per lane took 215
+    // VGPRs, 2 waves per SIMD).  FP64 with one source per lane (94 VGPRs, 5 waves)
+    // ran slower, 30.8 -> 33.4 ms at N = 2^22: twice the LDS reads per pair.
+    constexpr int NG = 2, TL = 64 / NG;
     extern __shared__ double4 adj_lds[];  // sized by the launch: blockDim.x entries each
     const int BT = blockDim.x, tid = threadIdx.x;
     double4 *tx = adj_lds;
@@ -1339,7 +1373,7 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
 #pragma unroll
                 for (int d = 0; d < NA; ++d) {
                     if (NG >= 4) acc[q][d] += __shfl_xor(acc[q][d], 16);
-                    acc[q][d] += __shfl_xor(acc[q][d], 32);
+                    if (NG >= 2) acc[q][d] += __shfl_xor(acc[q][d], 32);
                 }
 #pragma unroll
             for (int d = 0; d < NA; ++d) {
@@ -1524,18 +1558,31 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
         int bt = ((S.maxleaf + 63) / 64) * 64;
         bt = std::max(64, std::min(bt, PT));
         const size_t v4 = mixed ? sizeof(float4) : sizeof(double4);
-        const size_t lds_t = (size_t)bt * (sizeof(double4) + v4);
-        const size_t lds_s = (size_t)bt * (sizeof(double4) + 2 * v4);
+        const size_t per_t = sizeof(double4) + v4, per_s = sizeof(double4) + 2 * v4;
+        // target side: tiles of up to 4 blocks of sources, within 12 KB of LDS per wave
+        // (so that LDS does not cap the 4 waves per SIMD the registers allow)
+        auto tcap = [&](size_t per) {
+            int tc = bt;
+            while (tc + bt <= 4 * bt && (size_t)(tc + bt) * per <= (size_t)(12 << 10) * (bt / 64))
+                tc += bt;
+            return tc;
+        };
+        // (the source side keeps one leaf per tile: packing leaves there raised its
+        // registers from 136/156 to 166/240, mixed/FP64)
+        const int tc_t = tcap(per_t);
+        const size_t lds_t = (size_t)tc_t * per_t, lds_s = (size_t)bt * per_s;
         if (mixed) {
             hipLaunchKernelGGL(k_adj_p2p_t<true>, dim3(nleaf), dim3(bt), lds_t, sa, leaves,
-                               c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, lo, cnt, sp8, ob, gx2);
+                               c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, lo, cnt, sp8, ob, gx2,
+                               tc_t);
             FMM_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_adj_p2p_s<true>, dim3(nleaf), dim3(bt), lds_s, sa, leaves,
                                c.d_p2pT_off.as<int32_t>(), c.p2pT_sorted, c.sb, lo, cnt, sp8, ob,
                                gx2, gp2);
         } else {
             hipLaunchKernelGGL(k_adj_p2p_t<false>, dim3(nleaf), dim3(bt), lds_t, sa, leaves,
-                               c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, lo, cnt, sp8, ob, gx2);
+                               c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, lo, cnt, sp8, ob, gx2,
+                               tc_t);
             FMM_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_adj_p2p_s<false>, dim3(nleaf), dim3(bt), lds_s, sa, leaves,
                                c.d_p2pT_off.as<int32_t>(), c.p2pT_sorted, c.sb, lo, cnt, sp8, ob,

@@ -6,7 +6,10 @@
 // for i in t and j in s (the self pair included: R = 0 gives K = 0).
 //
 // One workgroup owns one target leaf, walks the leaf's CSR source list and
-// stages each source leaf in LDS (x, y, z in FP64; gamma, px, py, pz).  A wave
+// stages the source leaves in LDS (x, y, z in FP64; gamma, px, py, pz), several
+// consecutive leaves per tile (up to 4 times the block size, 10 KB per wave), so that
+// the per-tile work (barriers, the FP32 tile sums' compensated update) is shared
+// by more pairs.  A wave
 // covers 16 TPL targets as 16 target lanes x 4 source groups: lane (ti, g) holds
 // the TPL targets ti + 16 q and takes the staged sources j = g (mod 4), so one
 // LDS read serves TPL pairs (one target per lane moves 40 bytes of LDS data per
@@ -26,6 +29,9 @@
 namespace fmm4rb {
 
 constexpr int P2P_MAXT = 512;  // threads per block: up to 8 waves
+constexpr int P2P_TILE_MUL = 4;        // staged sources per tile: up to 4 x the block size
+constexpr size_t P2P_TILE_LDS_WAVE = 10 << 10;  // and at most 10 KB of LDS per wave (so
+                                                 // that LDS allows 4 waves per SIMD)
 
 template <typename T>
 __device__ inline void p2p_pair(const double *xi, double sx, double sy, double sz, T g, T px,
@@ -52,16 +58,17 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
     const int32_t *__restrict__ leaves, const int32_t *__restrict__ off,
     const uint64_t *__restrict__ keys, int sb, const int32_t *__restrict__ lo,
     const int32_t *__restrict__ cnt, const double *__restrict__ sp8, const float *__restrict__ sp4f,
-    double *__restrict__ nearf) {
+    double *__restrict__ nearf, int TC) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using V4 = typename std::conditional<MIXED, float4, double4>::type;
-    // LDS sized by the launch (blockDim.x entries each), not by P2P_MAXT: a one-wave
-    // block then holds 3 KB (mixed) and LDS does not limit the waves per CU
+    // LDS sized by the launch (TC entries each, a multiple of blockDim.x), not by
+    // P2P_MAXT: a one-wave block holds 10 KB (mixed, 256 sources), so LDS allows 4
+    // waves per SIMD, as the registers do
     extern __shared__ double2 p2p_lds[];
-    const int BT = blockDim.x;
+    const int BT = blockDim.x;                                     // TC: tile capacity
     double2 *txy = p2p_lds;                                        // source x, y (FP64)
-    double *tz = reinterpret_cast<double *>(p2p_lds + BT);         // source z
-    V4 *tq = reinterpret_cast<V4 *>(p2p_lds + BT + (BT + 1) / 2);  // gamma, px, py, pz
+    double *tz = reinterpret_cast<double *>(p2p_lds + TC);         // source z
+    V4 *tq = reinterpret_cast<V4 *>(p2p_lds + TC + (TC + 1) / 2);  // gamma, px, py, pz
     const int tid = threadIdx.x;
     const int lane = tid & 63, ti = lane & 15, g = lane >> 4;
     const int tw = (tid >> 6) * 16 * TPL;  // first target of this wave within a pass
@@ -91,59 +98,79 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
                 else acc[q][m] = 0.0;
             }
         }
+        // The tile's m staged sources against the lane's TPL targets; the tile's
+        // FP32 sums go into the Kahan pairs (MIXED) or straight into the sums.
+        auto run_tile = [&](int m) {
+            if constexpr (MIXED) {
+                T part[TPL][6];
+#pragma unroll
+                for (int q = 0; q < TPL; ++q)
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) part[q][k] = 0;
+                for (int jj = g; jj < m; jj += 4) {
+                    const double2 xy = txy[jj];
+                    const double zs = tz[jj];
+                    const V4 sq = tq[jj];  // gamma, px, py, pz
+#pragma unroll
+                    for (int q = 0; q < TPL; ++q)
+                        p2p_pair<T>(xi[q], xy.x, xy.y, zs, sq.x, sq.y, sq.z, sq.w, part[q]);
+                }
+#pragma unroll
+                for (int q = 0; q < TPL; ++q)
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {  // Kahan: hi - cmp is the running sum
+                        const float y = part[q][k] - cmp[q][k];
+                        const float t = hi[q][k] + y;
+                        cmp[q][k] = (t - hi[q][k]) - y;
+                        hi[q][k] = t;
+                    }
+            } else {
+                for (int jj = g; jj < m; jj += 4) {
+                    const double2 xy = txy[jj];
+                    const double zs = tz[jj];
+                    const V4 sq = tq[jj];
+#pragma unroll
+                    for (int q = 0; q < TPL; ++q)
+                        p2p_pair<T>(xi[q], xy.x, xy.y, zs, sq.x, sq.y, sq.z, sq.w, acc[q]);
+                }
+            }
+        };
+        // Fill the tile with consecutive source leaves (a leaf larger than the room
+        // left is split across tiles); fill is the same for every thread.
+        int fill = 0;
+        __syncthreads();  // the previous pass's last tile has been read
         for (int32_t e = beg; e < end; ++e) {
             const int32_t s = (int32_t)(keys[e] & mask);
             const int32_t sa = lo[s], cs = cnt[s];
-            for (int s0 = 0; s0 < cs; s0 += BT) {
-                __syncthreads();
-                const int j = s0 + tid;
-                if (j < cs) {
-                    const double4 x4 = p8[2 * (int64_t)(sa + j)];
-                    txy[tid] = make_double2(x4.x, x4.y);
-                    tz[tid] = x4.z;
+            for (int s0 = 0; s0 < cs;) {
+                const int take = min(cs - s0, TC - fill);
+#pragma unroll 1
+                for (int j = tid; j < take; j += BT) {
+                    const int64_t src = sa + s0 + j;
+                    const int slot = fill + j;
+                    const double4 x4 = p8[2 * src];
+                    txy[slot] = make_double2(x4.x, x4.y);
+                    tz[slot] = x4.z;
                     if constexpr (MIXED) {
-                        tq[tid] = q4[sa + j];
+                        tq[slot] = q4[src];
                     } else {
-                        const double4 p4 = p8[2 * (int64_t)(sa + j) + 1];  // px, py, pz, 0
-                        tq[tid] = make_double4(x4.w, p4.x, p4.y, p4.z);
+                        const double4 p4 = p8[2 * src + 1];  // px, py, pz, 0
+                        tq[slot] = make_double4(x4.w, p4.x, p4.y, p4.z);
                     }
                 }
-                __syncthreads();
-                const int m = min(BT, cs - s0);
-                if constexpr (MIXED) {
-                    T part[TPL][6];
-#pragma unroll
-                    for (int q = 0; q < TPL; ++q)
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) part[q][k] = 0;
-                    for (int jj = g; jj < m; jj += 4) {
-                        const double2 xy = txy[jj];
-                        const double zs = tz[jj];
-                        const V4 sq = tq[jj];  // gamma, px, py, pz
-#pragma unroll
-                        for (int q = 0; q < TPL; ++q)
-                            p2p_pair<T>(xi[q], xy.x, xy.y, zs, sq.x, sq.y, sq.z, sq.w, part[q]);
-                    }
-#pragma unroll
-                    for (int q = 0; q < TPL; ++q)
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) {  // Kahan: hi - cmp is the running sum
-                            const float y = part[q][k] - cmp[q][k];
-                            const float t = hi[q][k] + y;
-                            cmp[q][k] = (t - hi[q][k]) - y;
-                            hi[q][k] = t;
-                        }
-                } else {
-                    for (int jj = g; jj < m; jj += 4) {
-                        const double2 xy = txy[jj];
-                        const double zs = tz[jj];
-                        const V4 sq = tq[jj];
-#pragma unroll
-                        for (int q = 0; q < TPL; ++q)
-                            p2p_pair<T>(xi[q], xy.x, xy.y, zs, sq.x, sq.y, sq.z, sq.w, acc[q]);
-                    }
+                fill += take;
+                s0 += take;
+                if (fill == TC) {
+                    __syncthreads();
+                    run_tile(TC);
+                    __syncthreads();
+                    fill = 0;
                 }
             }
+        }
+        if (fill > 0) {
+            __syncthreads();
+            run_tile(fill);
         }
         // Sum the 4 source groups (lanes ti + 16 g), fixed order, in FP64.
         double tot[TPL][6];
@@ -192,12 +219,17 @@ void launch_p2p(Ctx &c, const int32_t *off, const uint64_t *keys, int sb, hipStr
     bt = std::max(64, std::min(bt, P2P_MAXT));
     // per source: x, y (16 B), z (8 B), gamma and p (16 B mixed, 32 B FP64): 40 B read per
     // source and lane in mixed precision instead of 48 with a padded double4
-    const size_t lds = (size_t)bt * sizeof(double2) + (size_t)((bt + 1) / 2) * sizeof(double2) +
-                       (size_t)bt * (mixed ? sizeof(float4) : sizeof(double4));
+    const size_t per = sizeof(double2) + sizeof(double) + (mixed ? sizeof(float4) : sizeof(double4));
+    int tc = bt;  // a multiple of the block size, >= 1 block
+    while (tc + bt <= P2P_TILE_MUL * bt &&
+           (size_t)(tc + bt) * per + 16 <= P2P_TILE_LDS_WAVE * (size_t)(bt / 64))
+        tc += bt;
+    const size_t lds = (size_t)tc * sizeof(double2) + (size_t)((tc + 1) / 2) * sizeof(double2) +
+                       (size_t)tc * (mixed ? sizeof(float4) : sizeof(double4));
 #define P2P_L(MX, TP)                                                                            \
     hipLaunchKernelGGL((k_p2p<MX, TP>), dim3(nleaf), dim3(bt), lds, st,                          \
                        c.d_leaves.as<int32_t>(), off, keys, sb, c.d_lo.as<int32_t>(),             \
-                       c.d_cnt.as<int32_t>(), c.d_sp8.as<double>(), q4, c.d_near.as<double>())
+                       c.d_cnt.as<int32_t>(), c.d_sp8.as<double>(), q4, c.d_near.as<double>(), tc)
     if (mixed) {
         if (tpl == 4) P2P_L(true, 4);
         else P2P_L(true, 8);
