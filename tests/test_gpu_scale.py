@@ -1,7 +1,9 @@
-"""Size-independent properties at the full configurations (no oracle run):
-the tree's ranges and boxes, the every-pair-exactly-once checksum of the
+"""The full configurations: the N=2^20 bunch against the oracle (bit-exact tree
+and lists, FP64 fields within 1e-12), and size-independent properties: the
+tree's ranges and boxes, the every-pair-exactly-once checksum of the
 interaction lists, FMM accuracy against the GPU BruteForce on sampled
-targets, mixed vs FP64, and determinism."""
+targets, mixed vs FP64, the reverse mode against finite differences, and
+determinism."""
 import numpy as np
 import pytest
 
