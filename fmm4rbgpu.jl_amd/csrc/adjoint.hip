@@ -1082,8 +1082,9 @@ __device__ inline T pair_geo(double R0d, double R1d, double R2d, T px, T py, T p
 // Target side: gx_i += sum_j J^T u_ij,  u_ij = g_j e_i + b_i x p_j.
 // One staged tile for the lane's PNG targets; DIAG: the tile holds the targets'
 // own leaf, and the self pair (source k == selfk + 16 q) is skipped.
-// NT targets per lane, 64 / NT lanes per group.
-template <bool DIAG, int NT, typename T, typename V4>
+// NT targets per lane, 64 / NT lanes per group; the first Q of them (the last wave of
+// a leaf may have fewer targets than 16 NT).
+template <bool DIAG, int NT, int Q, typename T, typename V4>
 __device__ __forceinline__ void adj_t_tile(const double4 *sx, const V4 *sq, int g, int m,
                                            int selfk, const double (&xi)[NT][3],
                                            const T (&e)[NT][3], const T (&b)[NT][3],
@@ -1092,7 +1093,7 @@ __device__ __forceinline__ void adj_t_tile(const double4 *sx, const V4 *sq, int 
         const double4 xs = sx[k];
         const V4 sqk = sq[k];  // gamma, px, py, pz
 #pragma unroll
-        for (int q = 0; q < NT; ++q) {
+        for (int q = 0; q < Q; ++q) {
             T R[3], pr;
             const bool self = DIAG && k == selfk + (64 / NT) * q;
             const T y = pair_geo<T>(xi[q][0] - xs.x, xi[q][1] - xs.y, xi[q][2] - xs.z, sqk.y,
@@ -1193,8 +1194,19 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ le
             T part[NT][3];
 #pragma unroll
             for (int q = 0; q < NT; ++q) part[q][0] = part[q][1] = part[q][2] = 0;
-            if (self) adj_t_tile<true, NT>(sx, sq, g, fill, selfk, xi, e, b, part);
-            else adj_t_tile<false, NT>(sx, sq, g, fill, 0, xi, e, b, part);
+            // targets of this wave in this pass: 16 nq slots (wave-uniform)
+            const int nq = max(0, min(NT, (ct - tb - wbase + TL - 1) / TL));
+            if (self) {
+                if (nq == NT) adj_t_tile<true, NT, NT>(sx, sq, g, fill, selfk, xi, e, b, part);
+                else if (nq == 3) adj_t_tile<true, NT, 3>(sx, sq, g, fill, selfk, xi, e, b, part);
+                else if (nq == 2) adj_t_tile<true, NT, 2>(sx, sq, g, fill, selfk, xi, e, b, part);
+                else if (nq == 1) adj_t_tile<true, NT, 1>(sx, sq, g, fill, selfk, xi, e, b, part);
+            } else {
+                if (nq == NT) adj_t_tile<false, NT, NT>(sx, sq, g, fill, 0, xi, e, b, part);
+                else if (nq == 3) adj_t_tile<false, NT, 3>(sx, sq, g, fill, 0, xi, e, b, part);
+                else if (nq == 2) adj_t_tile<false, NT, 2>(sx, sq, g, fill, 0, xi, e, b, part);
+                else if (nq == 1) adj_t_tile<false, NT, 1>(sx, sq, g, fill, 0, xi, e, b, part);
+            }
 #pragma unroll
             for (int q = 0; q < NT; ++q)
 #pragma unroll
