@@ -12,7 +12,10 @@
 namespace fmm4rb {
 
 constexpr int RS_THREADS = 256;  // 4 waves of 64
-constexpr int RS_ITEMS = 8;
+// keys per thread: a 6144-key tile writes each digit's run ~24 keys long (with 2048-key
+// tiles, ~8 keys: partial HBM lines); the tree's presort at N = 25.6M 27.2 -> 25.5 ms
+// (2048 -> 6144; 8192-key tiles hold a workgroup per CU and were slower)
+constexpr int RS_ITEMS = 24;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_BINS = 256;
 
@@ -56,26 +59,52 @@ __device__ inline uint32_t block_incl_scan256(uint32_t v, uint32_t *s) {
 }
 
 // Exclusive scan of each row (len entries) of rows[gridDim.x*gridDim.y][len];
-// the row total goes to tot[row].
-__global__ __launch_bounds__(RS_THREADS) void k_scan_rows(uint32_t *__restrict__ rows, int len,
-                                                          uint32_t *__restrict__ tot) {
-    __shared__ uint32_t s[RS_THREADS];
-    __shared__ uint32_t chunk;
+// the row total goes to tot[row].  Thread t loads SC_I consecutive entries at once
+// (independent loads), and one block-wide scan of the threads' sums covers ST * SC_I
+// entries: a row of 12,500 tile counts (the tree's partition at N = 25.6M) takes one
+// step instead of 49 barrier-bound 256-entry steps.
+constexpr int SC_I = 16;
+template <int ST>
+__global__ __launch_bounds__(ST) void k_scan_rows(uint32_t *__restrict__ rows, int len,
+                                                  uint32_t *__restrict__ tot) {
+    __shared__ uint32_t ws[ST / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t row = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
     uint32_t *r = rows + row * len;
     uint32_t carry = 0;
-    for (int base = 0; base < len; base += RS_THREADS) {
-        int i = base + threadIdx.x;
-        uint32_t v = i < len ? r[i] : 0u;
-        uint32_t incl = block_incl_scan256(v, s);
-        if (i < len) r[i] = carry + incl - v;
-        // the last thread's inclusive value is the chunk total
-        if (threadIdx.x == RS_THREADS - 1) chunk = incl;
+    for (int base = 0; base < len; base += ST * SC_I) {
+        const int b0 = base + tid * SC_I;
+        uint32_t v[SC_I], s = 0;
+#pragma unroll
+        for (int i = 0; i < SC_I; ++i) {
+            v[i] = b0 + i < len ? r[b0 + i] : 0u;
+            s += v[i];
+        }
+        uint32_t x = s;  // inclusive scan over the wave, then the waves' totals
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) ws[wave] = x;
         __syncthreads();
-        carry += chunk;
-        __syncthreads();
+        uint32_t pre = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < ST / 64; ++w) {
+            const uint32_t t = ws[w];
+            pre += w < wave ? t : 0u;
+            total += t;
+        }
+        uint32_t run = carry + pre + (x - s);
+#pragma unroll
+        for (int i = 0; i < SC_I; ++i) {
+            if (b0 + i < len) r[b0 + i] = run;
+            run += v[i];
+        }
+        carry += total;
+        __syncthreads();  // ws is rewritten by the next step
     }
-    if (threadIdx.x == 0) tot[row] = carry;
+    if (tid == 0) tot[row] = carry;
 }
 
 // Stable in-tile ranking: each wave owns RS_ITEMS x 64 consecutive keys of the tile
@@ -174,9 +203,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     }
 }
 
-void scan_rows(uint32_t *rows, int nrows, int len, uint32_t *tot, hipStream_t st) {
-    hipLaunchKernelGGL(k_scan_rows, dim3(nrows), dim3(RS_THREADS), 0, st, rows, len, tot);
+// rows[nrows] of len words (a grid of gx x gy rows when gy > 1)
+static void scan_rows_grid(uint32_t *rows, dim3 grid, int len, uint32_t *tot, hipStream_t st) {
+    if (len > 256 * SC_I)
+        hipLaunchKernelGGL(k_scan_rows<1024>, grid, dim3(1024), 0, st, rows, len, tot);
+    else
+        hipLaunchKernelGGL(k_scan_rows<256>, grid, dim3(256), 0, st, rows, len, tot);
     FMM_LAUNCH_CHECK();
+}
+
+void scan_rows(uint32_t *rows, int nrows, int len, uint32_t *tot, hipStream_t st) {
+    scan_rows_grid(rows, dim3(nrows), len, tot, st);
 }
 
 bool radix_sort(uint64_t *keys, uint64_t *keys_alt, uint32_t *vals, uint32_t *vals_alt, int64_t n,
@@ -192,9 +229,7 @@ bool radix_sort(uint64_t *keys, uint64_t *keys_alt, uint32_t *vals, uint32_t *va
         dim3 grid(nblk, nseg);
         hipLaunchKernelGGL(k_rs_hist, grid, dim3(RS_THREADS), 0, st, ki, n, shift, hist, nblk);
         FMM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_scan_rows, dim3(RS_BINS, nseg), dim3(RS_THREADS), 0, st, hist, nblk,
-                           dtot);
-        FMM_LAUNCH_CHECK();
+        scan_rows_grid(hist, dim3(RS_BINS, nseg), nblk, dtot, st);
         if (vals)
             hipLaunchKernelGGL(k_rs_scatter<true>, grid, dim3(RS_THREADS), 0, st, ki, vi, ko, vo, n,
                                shift, hist, nblk, dtot);
