@@ -399,10 +399,10 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
                                hipMemcpyDeviceToHost, st));
         FMM_HIP(hipStreamSynchronize(st));
     }
-    launch_l2l(c);
+    launch_l2l(c, false);
     FMM_HIP(hipStreamWaitEvent(st, c.evx[2], 0));
     FMM_HIP(hipEventRecord(c.ev[6], st));
-    launch_l2p(c, amp, E, B);
+    launch_leaf_l2p(c, amp, E, B);
     FMM_HIP(hipEventRecord(c.ev[7], st));
     unsigned long long ls[2] = {0, 0};
     if (stats) {

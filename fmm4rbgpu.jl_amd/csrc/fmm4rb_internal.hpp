@@ -71,6 +71,16 @@ __device__ inline double rsqrt_nr(double d) {
     const double r = fma(-d * y, y, 1.0);
     return fma(y * 0.5, r, y);
 }
+// d^-3/2, the kernel's scale, as y^3 with y = 1/sqrt(d): FP64 rsqrt_nr, FP32
+// v_rsq_f32.  (y times v_rcp_f32(d) saves a multiply but ran slower: the second
+// transcendental does not issue for free, M2L 15.7 -> 17.5 ms at N = 2^22.)
+template <typename T>
+__device__ inline T rsq3(T d) {
+    T y;
+    if constexpr (sizeof(T) == 4) y = __builtin_amdgcn_rsqf(d);
+    else y = rsqrt_nr(d);
+    return (y * y) * y;
+}
 constexpr int MAX_N0 = 1024;
 
 // Workgroup b of a launch runs on XCD b % 8 (round-robin dispatch over the
@@ -137,8 +147,10 @@ void sort_pairs_csr(Ctx &c, uint64_t *keys, uint64_t *keys_alt, int64_t n, int32
 void launch_node_geo(Ctx &c);
 void launch_upward(Ctx &c, hipStream_t st);
 void launch_downward(Ctx &c, double amp, double *E, double *B);
-void launch_l2l(Ctx &c);
-void launch_l2p(Ctx &c, double amp, double *E, double *B);
+// L2L level by level from the root; leaves = false skips the leaves, whose transfer
+// k_leaf_l2p does (launch_leaf_l2p: the leaves' final locals, then E and B).
+void launch_l2l(Ctx &c, bool leaves = true);
+void launch_leaf_l2p(Ctx &c, double amp, double *E, double *B);
 void launch_mp_to_f32(Ctx &c, hipStream_t st);
 
 // ---- interactions (m2l.hip, p2p.hip)

@@ -54,6 +54,36 @@ __device__ inline void lgw(double x, const double *nodes, double *w) {
     for (int i = 0; i < NP; ++i) w[i] = w[i] / sum;
 }
 
+// 1/v in FP64: the hardware estimate v_rcp_f64 refined by two Newton steps.
+__device__ inline double rcp_nr(double v) {
+    double r = __builtin_amdgcn_rcp(v);
+    r = fma(r, fma(-v, r, 1.0), r);
+    return fma(r, fma(-v, r, 1.0), r);
+}
+
+// lgw with reciprocals instead of divisions (within a few ulps of it): the particle
+// weights of P2M and L2P, which are latency-bound on the division sequences.
+template <int NP>
+__device__ inline void lgw_rcp(double x, const double *nodes, double *w) {
+    int flag = -1;
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+        if (fabs(x - nodes[i]) < EPS64) flag = i;
+    if (flag >= 0) {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) w[i] = (i == flag) ? 1.0 : 0.0;
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < NP; ++i) w[i] = bary<NP>(i) * rcp_nr(x - nodes[i]);
+    double sum = w[0];
+#pragma unroll
+    for (int i = 1; i < NP; ++i) sum = sum + w[i];
+    const double inv = rcp_nr(sum);
+#pragma unroll
+    for (int i = 0; i < NP; ++i) w[i] = w[i] * inv;
+}
+
 // Per-node centre, admissibility radius and diameter (interaction_lists.jl:15-19,
 // bbox.jl:9-11) and the Chebyshev nodes of each axis (cheb2.jl:3-5).
 __global__ void k_node_geo(int64_t nc, const uint8_t *__restrict__ flags,
@@ -84,6 +114,10 @@ __global__ void k_node_geo(int64_t nc, const uint8_t *__restrict__ flags,
 
 // ------------------------------------------------------------------- P2M
 constexpr int OP_T = 128;
+
+// The staged particles' weights per axis, W[axis][node][slot], rows padded by one
+// double so that lanes reading different nodes of one slot hit different banks.
+constexpr int OP_TP = OP_T + 1;
 
 template <int NP>
 __global__ __launch_bounds__(OP_T) void k_p2m(const int32_t *__restrict__ leaves,
@@ -292,46 +326,204 @@ __global__ __launch_bounds__(OP_T) void k_l2l(const int32_t *__restrict__ nodes,
 }
 
 // ---------------------------------------------------- L2P and final sum
+// One leaf per workgroup: the leaf's final local expansion is its M2L sum plus its
+// parent's final expansion transferred down (the same transfer as k_l2l), kept in
+// LDS for the evaluation at the leaf's particles (downwardpass.jl:47-80) and also
+// stored (the reverse mode's pullback reads it); then E and B, near plus far field
+// scaled by amp (update_particles_field.jl:142-147), go to the particle's original
+// index.  The kernel waits on memory, not on arithmetic, so every global load of
+// the first particle pass (both expansions, coordinates, near field, indices) is
+// issued at the start, before the transfer.  Per particle the weights factor per
+// axis: for every (j, k) row the NP values along x are contracted with wx first,
+// then weighted by wy wz.  When the leaf fits half the block, two lanes share a
+// particle, each taking half of the rows, and their sums meet through LDS.
 template <int NP>
-__global__ __launch_bounds__(OP_T) void k_l2p(const int32_t *__restrict__ leaves,
-                                              const int32_t *__restrict__ lo,
-                                              const int32_t *__restrict__ cnt,
-                                              const int32_t *__restrict__ parind,
-                                              const double *__restrict__ sp8,
-                                              const double *__restrict__ ncoord,
-                                              const double *__restrict__ loc,
-                                              const double *__restrict__ nearf, double amp,
-                                              double *__restrict__ E, double *__restrict__ B) {
-    constexpr int NN = NP * NP * NP;
-    __shared__ double Ls[NN * 6];
-    __shared__ double nd[3 * NP];
+__global__ __launch_bounds__(OP_T) void k_leaf_l2p(const int32_t *__restrict__ leaves,
+                                                   const int32_t *__restrict__ lo,
+                                                   const int32_t *__restrict__ cnt,
+                                                   const int32_t *__restrict__ parind,
+                                                   const double *__restrict__ sp8,
+                                                   const double *__restrict__ ncoord,
+                                                   double *__restrict__ loc,
+                                                   const double *__restrict__ nearf, double amp,
+                                                   double *__restrict__ E, double *__restrict__ B) {
+    constexpr int NN = NP * NP * NP, NR = NP * NP, NE = 6 * NN;
+    constexpr int CG = TransferGroup<NP, 6>::CG;
+    // the expansions' elements e = tid + OP_T q in registers (all six components in
+    // one transfer pass)
+    constexpr bool PRE = CG == 6 && NE <= 8 * OP_T;
+    constexpr int EPT = PRE ? (NE + OP_T - 1) / OP_T : 1;
+    constexpr int NBUF = 3 * CG * NN, NWB = 3 * NP * OP_TP + (OP_T / 2) * 6;
+    __shared__ double Ls[NE];
+    __shared__ double sm[NBUF > NWB ? NBUF : NWB];  // transfer buffers, then weights
+    __shared__ double Wm[3][NP][NP];                 // [axis][child node][parent node]
+    __shared__ double pn[3 * NP], nd[3 * NP];
+    double (*bin)[NN] = reinterpret_cast<double (*)[NN]>(sm);
+    double (*ba)[NN] = reinterpret_cast<double (*)[NN]>(sm + CG * NN);
+    double (*bb)[NN] = reinterpret_cast<double (*)[NN]>(sm + 2 * CG * NN);
+    double (*W)[NP][OP_TP] = reinterpret_cast<double (*)[NP][OP_TP]>(sm);
+    double (*red)[6] = reinterpret_cast<double (*)[6]>(sm + 3 * NP * OP_TP);
     const int tid = threadIdx.x;
-    const int32_t k = leaves[blockIdx.x];
-    const int32_t a = lo[k], c = cnt[k];
-    for (int e = tid; e < NN * 6; e += OP_T) Ls[e] = loc[(int64_t)k * NN * 6 + e];
-    if (tid < 3 * NP) nd[tid] = ncoord[(int64_t)k * 3 * NP + tid];
+    const int32_t C = leaves[blockIdx.x];
+    const int32_t a = lo[C], c = cnt[C];
+    const int32_t P = C > 0 ? (C - 1) / 2 : 0;
+    double *lc = loc + (int64_t)C * NE;
+    const double *lp = loc + (int64_t)P * NE;
+    const int PT = 2 * c <= OP_T ? OP_T / 2 : OP_T;  // particles per pass
+    const int slot = tid % PT, part = tid / PT;
+    // ---- loads of the first pass
+    double vl[EPT], vp[EPT];
+    if constexpr (PRE) {
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = tid + OP_T * q;
+            vl[q] = e < NE ? lc[e] : 0.0;
+            vp[q] = (C > 0 && e < NE) ? lp[e] : 0.0;
+        }
+    }
+    if (tid < 3 * NP) {
+        nd[tid] = ncoord[(int64_t)C * 3 * NP + tid];
+        if (C > 0) pn[tid] = ncoord[(int64_t)P * 3 * NP + tid];
+    }
+    // weight tasks (particle, axis) of the first pass: e = tid + OP_T u < 3 m
+    const int m0 = min(PT, c);
+    double xt[3];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        const int e = tid + OP_T * u, d = e / max(m0, 1);
+        xt[u] = e < 3 * m0 ? sp8[8 * ((int64_t)a + e - d * m0) + d] : 0.0;
+    }
+    double nf[6];
+    int64_t qo = 0;
+    if (part == 0 && slot < m0) {
+        const int64_t i = (int64_t)a + slot;
+        qo = parind[i];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) nf[d] = nearf[6 * i + d];
+    }
+    // ---- final local expansion of the leaf
     __syncthreads();
-    for (int p = tid; p < c; p += OP_T) {
-        const int64_t i = a + p;
-        const double *x = sp8 + 8 * i;
-        double wx[NP], wy[NP], wz[NP];
-        lgw<NP>(x[0], nd, wx);
-        lgw<NP>(x[1], nd + NP, wy);
-        lgw<NP>(x[2], nd + 2 * NP, wz);
-        double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int kk = 0; kk < NP; ++kk)
-            for (int j = 0; j < NP; ++j)
+    if (C == 0) {  // the root is the only leaf: no parent
+        for (int e = tid; e < NE; e += OP_T) Ls[e] = lc[e];
+    } else {
+        if (tid < 3 * NP) {
+            const int d = tid / NP, ci = tid % NP;
+            lgw<NP>(nd[tid], pn + d * NP, &Wm[d][ci][0]);
+        }
+        for (int c0 = 0; c0 < 6; c0 += CG) {
+            if constexpr (PRE) {
 #pragma unroll
-                for (int ii = 0; ii < NP; ++ii) {
-                    const double w = (wx[ii] * wy[j]) * wz[kk];
-                    const double *l = Ls + 6 * (ii + NP * (j + NP * kk));
-#pragma unroll
-                    for (int d = 0; d < 6; ++d) f[d] += l[d] * w;
+                for (int q = 0; q < EPT; ++q) {
+                    const int e = tid + OP_T * q;
+                    if (e < NE) bin[e % 6][e / 6] = vp[q];
                 }
-        const int64_t q = parind[i];
-        for (int d = 0; d < 3; ++d) {
-            E[3 * q + d] = (nearf[6 * i + d] + f[d]) * amp;
-            B[3 * q + d] = (nearf[6 * i + 3 + d] + f[3 + d]) * amp;
+            } else {
+                __syncthreads();
+                for (int e = tid; e < CG * NN; e += OP_T) {
+                    const int n = e / CG, cc = e % CG;
+                    bin[cc][n] = lp[(int64_t)n * 6 + c0 + cc];
+                }
+            }
+            __syncthreads();
+            for (int e = tid; e < CG * NN; e += OP_T) {  // ba[ci, Pj, Pk]
+                const int cc = e / NN, n = e % NN, ci = n % NP, r = n / NP;
+                double s = 0.0;
+                for (int Pi = 0; Pi < NP; ++Pi) s += Wm[0][ci][Pi] * bin[cc][Pi + NP * r];
+                ba[cc][n] = s;
+            }
+            __syncthreads();
+            for (int e = tid; e < CG * NN; e += OP_T) {  // bb[ci, cj, Pk]
+                const int cc = e / NN, n = e % NN;
+                const int ci = n % NP, cj = (n / NP) % NP, Pk = n / (NP * NP);
+                double s = 0.0;
+                for (int Pj = 0; Pj < NP; ++Pj) s += Wm[1][cj][Pj] * ba[cc][ci + NP * (Pj + NP * Pk)];
+                bb[cc][n] = s;
+            }
+            __syncthreads();
+            if constexpr (PRE) {  // M2L sum + contract z, element e = n * 6 + cc
+#pragma unroll
+                for (int q = 0; q < EPT; ++q) {
+                    const int e = tid + OP_T * q;
+                    if (e < NE) {
+                        const int n = e / 6, cc = e % 6;
+                        const int ij = n % (NP * NP), ck = n / (NP * NP);
+                        double s = 0.0;
+                        for (int Pk = 0; Pk < NP; ++Pk) s += Wm[2][ck][Pk] * bb[cc][ij + NP * NP * Pk];
+                        Ls[e] = vl[q] + s;
+                        lc[e] = vl[q] + s;
+                    }
+                }
+            } else {
+                for (int e = tid; e < CG * NN; e += OP_T) {
+                    const int n = e / CG, cc = e % CG;
+                    const int ij = n % (NP * NP), ck = n / (NP * NP);
+                    double s = 0.0;
+                    for (int Pk = 0; Pk < NP; ++Pk) s += Wm[2][ck][Pk] * bb[cc][ij + NP * NP * Pk];
+                    const int64_t o = (int64_t)n * 6 + c0 + cc;
+                    Ls[o] = lc[o] + s;
+                    lc[o] = Ls[o];
+                }
+                __syncthreads();  // bin is refilled by the next pass
+            }
+        }
+    }
+    // ---- L2P
+    const int r0 = PT == OP_T ? 0 : (part == 0 ? 0 : (NR + 1) / 2);
+    const int r1 = PT == OP_T ? NR : (part == 0 ? (NR + 1) / 2 : NR);
+    for (int p0 = 0; p0 < c; p0 += PT) {
+        const int m = min(PT, c - p0);
+        __syncthreads();  // Ls complete; the previous pass's weights and sums read
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int e = tid + OP_T * u;
+            if (e < 3 * m) {
+                const int d = e / m, sl = e - d * m;
+                const double x = p0 == 0 ? xt[u] : sp8[8 * ((int64_t)a + p0 + sl) + d];
+                double w[NP];
+                lgw_rcp<NP>(x, nd + d * NP, w);
+#pragma unroll
+                for (int i = 0; i < NP; ++i) W[d][i][sl] = w[i];
+            }
+        }
+        __syncthreads();
+        double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (slot < m) {
+            double wx[NP];
+#pragma unroll
+            for (int i = 0; i < NP; ++i) wx[i] = W[0][i][slot];
+#pragma unroll 1
+            for (int r = r0; r < r1; ++r) {
+                const double *l = Ls + 6 * NP * r;
+                double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int i = 0; i < NP; ++i)
+#pragma unroll
+                    for (int d = 0; d < 6; ++d) t[d] = fma(l[6 * i + d], wx[i], t[d]);
+                const double wyz = W[1][r % NP][slot] * W[2][r / NP][slot];
+#pragma unroll
+                for (int d = 0; d < 6; ++d) f[d] = fma(wyz, t[d], f[d]);
+            }
+        }
+        if (PT < OP_T) {  // second half of the rows into the first, fixed order
+            if (part == 1 && slot < m)
+#pragma unroll
+                for (int d = 0; d < 6; ++d) red[slot][d] = f[d];
+            __syncthreads();
+            if (part == 0 && slot < m)
+#pragma unroll
+                for (int d = 0; d < 6; ++d) f[d] += red[slot][d];
+        }
+        if (part == 0 && slot < m) {
+            const int64_t i = (int64_t)a + p0 + slot;
+            if (p0 > 0) {
+                qo = parind[i];
+#pragma unroll
+                for (int d = 0; d < 6; ++d) nf[d] = nearf[6 * i + d];
+            }
+            for (int d = 0; d < 3; ++d) {
+                E[3 * qo + d] = (nf[d] + f[d]) * amp;
+                B[3 * qo + d] = (nf[3 + d] + f[3 + d]) * amp;
+            }
         }
     }
 }
@@ -398,13 +590,16 @@ void launch_mp_to_f32(Ctx &c, hipStream_t st) {
     FMM_LAUNCH_CHECK();
 }
 
-void launch_l2l(Ctx &c) {
+void launch_l2l(Ctx &c, bool leaves) {
     const Shape &S = c.shape;
     hipStream_t st = c.stream;
     for (int l = 1; l <= S.ml; ++l) {
-        const int64_t b = S.lvl_off[l], e = S.lvl_off[l + 1];
+        // every node of level l, or only its split nodes (the leaves' transfer is
+        // then part of k_leaf_l2p)
+        const int64_t b = leaves ? S.lvl_off[l] : S.split_off[l];
+        const int64_t e = leaves ? S.lvl_off[l + 1] : S.split_off[l + 1];
         if (e == b) continue;
-        const int32_t *nodes = c.d_lvl_nodes.as<int32_t>() + b;
+        const int32_t *nodes = (leaves ? c.d_lvl_nodes : c.d_lvl_split).as<int32_t>() + b;
 #define L2L_L(NP)                                                                          \
     hipLaunchKernelGGL(k_l2l<NP>, dim3((unsigned)(e - b)), dim3(OP_T), 0, st, nodes,      \
                        c.d_ncoord.as<double>(), c.d_loc.as<double>())
@@ -413,21 +608,21 @@ void launch_l2l(Ctx &c) {
     }
 }
 
-void launch_l2p(Ctx &c, double amp, double *E, double *B) {
+void launch_leaf_l2p(Ctx &c, double amp, double *E, double *B) {
     const unsigned nleaf = (unsigned)c.shape.leaves.size();
     hipStream_t st = c.stream;
 #define L2P_L(NP)                                                                             \
-    hipLaunchKernelGGL(k_l2p<NP>, dim3(nleaf), dim3(OP_T), 0, st, c.d_leaves.as<int32_t>(),  \
-                       c.d_lo.as<int32_t>(), c.d_cnt.as<int32_t>(), c.d_parind.as<int32_t>(), \
-                       c.d_sp8.as<double>(), c.d_ncoord.as<double>(), c.d_loc.as<double>(),   \
-                       c.d_near.as<double>(), amp, E, B)
+    hipLaunchKernelGGL(k_leaf_l2p<NP>, dim3(nleaf), dim3(OP_T), 0, st,                       \
+                       c.d_leaves.as<int32_t>(), c.d_lo.as<int32_t>(), c.d_cnt.as<int32_t>(), \
+                       c.d_parind.as<int32_t>(), c.d_sp8.as<double>(), c.d_ncoord.as<double>(), \
+                       c.d_loc.as<double>(), c.d_near.as<double>(), amp, E, B)
     FMM_NP_SWITCH(c.np, L2P_L);
     FMM_LAUNCH_CHECK();
 }
 
 void launch_downward(Ctx &c, double amp, double *E, double *B) {
-    launch_l2l(c);
-    launch_l2p(c, amp, E, B);
+    launch_l2l(c, false);
+    launch_leaf_l2p(c, amp, E, B);
 }
 
 }  // namespace fmm4rb

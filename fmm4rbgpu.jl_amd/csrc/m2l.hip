@@ -45,12 +45,6 @@ struct M2LShape {
     static constexpr int TPT = (NN + BLK - 1) / BLK;
 };
 
-template <typename T>
-__device__ inline T rsq(T d);
-template <>
-__device__ inline double rsq<double>(double d) { return rsqrt_nr(d); }
-template <>
-__device__ inline float rsq<float>(float d) { return __builtin_amdgcn_rsqf(d); }
 
 // One source cluster's contribution to one target node.  Per node pair: the
 // geometry (3 operations on row-hoisted terms; QUAD: 2), d^-1/2, y3 and 8
@@ -116,8 +110,7 @@ __device__ inline void m2l_pair(double tx, double ty, double tz, const double *s
                     const T pr = fma(px, dx, prjk);
                     d = fma(pr, pr, rr);
                 }
-                const T y = rsq<T>(d);
-                const T y3 = (y * y) * y;
+                const T y3 = rsq3<T>(d);
                 const V4 m = row[si];
                 const T u = y3 * dx;
                 if (si == 0) {

@@ -43,10 +43,7 @@ __device__ inline void p2p_pair(const double *xi, double sx, double sy, double s
     const T rr = fma(R2, R2, fma(R1, R1, fma(R0, R0, (T)EPS64)));
     const T pr = fma(pz, R2, fma(py, R1, px * R0));
     const T d = fma(pr, pr, rr);
-    T y;
-    if constexpr (sizeof(T) == 4) y = __builtin_amdgcn_rsqf(d);
-    else y = rsqrt_nr(d);
-    const T y3 = (y * y) * y;
+    const T y3 = rsq3<T>(d);
     const T u0 = y3 * R0, u1 = y3 * R1, u2 = y3 * R2;  // K = d^-3/2 R
     a[0] = fma(g, u0, a[0]);                            // E += g K
     a[1] = fma(g, u1, a[1]);
