@@ -102,7 +102,7 @@ static void *scratch_ptr(int device) {
 constexpr int CTX_NBUF_MAX = 80;
 static int ctx_bufs(Ctx &c, DevBuf **out) {
     DevBuf *all[] = {&c.d_lo, &c.d_cnt, &c.d_flags, &c.d_leaves, &c.d_lvl_nodes, &c.d_lvl_split,
-            &c.d_leftbase, &c.d_part, &c.d_pavg, &c.d_bad, &c.d_keys, &c.d_keys_alt,
+            &c.d_leftbase, &c.d_part, &c.d_pavg, &c.d_bad, &c.d_tflag, &c.d_keys, &c.d_keys_alt,
             &c.d_vals, &c.d_vals_alt, &c.d_rs_scratch, &c.d_seg, &c.d_seg_alt,
             &c.d_rk, &c.d_pivot, &c.d_bsum, &c.d_axis, &c.d_parind, &c.d_leafof,
             &c.d_bbox, &c.d_geo, &c.d_ncoord, &c.d_costab, &c.d_ext, &c.d_sp8, &c.d_sp8f,
@@ -241,9 +241,12 @@ static void prepare(Ctx &c, int64_t N, int64_t N0, int n, int precision) {
     c.sb = S.ml + 1;
     const size_t nc = (size_t)S.nc, NN = (size_t)c.nn, n3 = 3 * (size_t)N;
     const int64_t nleaf = (int64_t)S.leaves.size();
-    c.d_part.ensure(4 * 1024 * sizeof(double));  // 3 sums and a digest word per block
-    c.d_pavg.ensure(5 * sizeof(double));         // p_avg, g_avg, input digest
+    // per block: 3 sums, a digest word and the position range
+    c.d_part.ensure(10 * 1024 * sizeof(double));
+    // p_avg, g_avg, input digest, the non-finite flag, 2 spare, position range
+    c.d_pavg.ensure(16 * sizeof(double));
     c.d_bad.ensure(sizeof(int));
+    c.d_tflag.ensure(sizeof(uint32_t));
     c.d_keys.ensure(n3 * 8);
     c.d_keys_alt.ensure(n3 * 8);
     c.d_vals.ensure(n3 * 4);
@@ -317,18 +320,28 @@ __global__ __launch_bounds__(256) void k_list_stats(const uint64_t *__restrict__
 }
 
 // p_avg and the input check; returns after the check (one host sync).
+// p_avg, g_avg, the input digest and the non-finite flag (words 0-5 of d_pavg) in
+// one copy.
 static void stage_pavg(Ctx &c, const double *pos, const double *mom) {
     hipStream_t st = c.stream;
-    zero_words(c.d_bad.p, 1, st);
-    launch_pavg(pos, mom, c.N, c.d_part.as<double>(), c.d_pavg.as<double>(), c.d_bad.as<int>(), st);
-    int bad = 0;
-    FMM_HIP(hipMemcpyAsync(&bad, c.d_bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(c.pavg_host, c.d_pavg.p, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipMemcpyAsync(&c.in_digest, c.d_pavg.as<double>() + 4, sizeof(uint64_t),
-                           hipMemcpyDeviceToHost, st));
+    double *pv = c.d_pavg.as<double>();
+    zero_words(pv + 5, 2, st);
+    launch_pavg(pos, mom, c.N, c.d_part.as<double>(), pv, reinterpret_cast<int *>(pv + 5), st);
+    double h[6];
+    FMM_HIP(hipMemcpyAsync(h, pv, sizeof(h), hipMemcpyDeviceToHost, st));
     FMM_HIP(hipStreamSynchronize(st));
+    for (int d = 0; d < 4; ++d) c.pavg_host[d] = h[d];
+    std::memcpy(&c.in_digest, &h[4], sizeof(uint64_t));
+    int bad = 0;
+    std::memcpy(&bad, &h[5], sizeof(int));
     if (bad) fail(FMM4RB_EINVAL, "positions or momenta contain NaN or Inf");
 }
+
+// Tree, node geometry, the upward pass (forked) and the lists.  The tree's presort
+// uses quantised keys; if it could not order a run of equal keys (reported at the
+// list build's synchronisation), tree and lists are built again with exact keys.
+static void tree_and_lists(Ctx &c, const double *pos, const double *mom, int64_t N, int64_t N0,
+                           double eta, hipEvent_t ev_tree);
 
 // The upward pass (P2M, M2M; for mixed precision also the FP32 copy of the
 // multipoles) needs only the tree: it runs on the third stream, beside the list
@@ -341,6 +354,18 @@ static void fork_upward(Ctx &c) {
     launch_upward(c, s3);
     if (c.precision == FMM4RB_MIXED) launch_mp_to_f32(c, s3);
     FMM_HIP(hipEventRecord(c.evu[2], s3));
+}
+
+static void tree_and_lists(Ctx &c, const double *pos, const double *mom, int64_t N, int64_t N0,
+                           double eta, hipEvent_t ev_tree) {
+    for (int exact = 0; exact < 2; ++exact) {
+        build_tree_device(c, pos, mom, N, N0, exact == 1);
+        launch_node_geo(c);
+        FMM_HIP(hipEventRecord(ev_tree, c.stream));
+        fork_upward(c);
+        if (build_lists_device(c, eta)) return;
+    }
+    fail(FMM4RB_EDEVICE, "tree presort: the exact keys left a run unordered");
 }
 
 // refit: keep the tree and lists of the last completed call with the same N, N0
@@ -364,11 +389,10 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
     FMM_HIP(hipEventRecord(c.ev[0], st));
     stage_pavg(c, pos, mom);
     FMM_HIP(hipEventRecord(c.ev[1], st));
-    if (refit) refit_tree_device(c, pos, mom);
-    else build_tree_device(c, pos, mom, N, N0);
-    launch_node_geo(c);
-    FMM_HIP(hipEventRecord(c.ev[2], st));
     if (refit) {
+        refit_tree_device(c, pos, mom);
+        launch_node_geo(c);
+        FMM_HIP(hipEventRecord(c.ev[2], st));
         // no list build to run beside: the upward pass goes first on this stream, so
         // that M2L starts together with P2P and the two share the GPU
         FMM_HIP(hipEventRecord(c.ev[3], st));
@@ -377,8 +401,7 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
         if (c.precision == FMM4RB_MIXED) launch_mp_to_f32(c, st);
         FMM_HIP(hipEventRecord(c.evu[2], st));
     } else {
-        fork_upward(c);
-        build_lists_device(c, eta);
+        tree_and_lists(c, pos, mom, N, N0, eta, c.ev[2]);
         FMM_HIP(hipEventRecord(c.ev[3], st));
     }
     // the near field needs only the tree order and the lists: it runs on the
@@ -490,11 +513,7 @@ static void run_vjp(Ctx &c, int64_t N, const double *pos, const double *mom, dou
     if (saved_locals && c.in_digest != c.loc_digest) fail(FMM4RB_EINVAL, nofwd);
     c.tree_valid = false;
     FMM_HIP(hipEventRecord(c.ev[1], st));
-    build_tree_device(c, pos, mom, N, N0);
-    launch_node_geo(c);
-    FMM_HIP(hipEventRecord(c.ev[2], st));
-    fork_upward(c);
-    build_lists_device(c, eta);
+    tree_and_lists(c, pos, mom, N, N0, eta, c.ev[2]);
     FMM_HIP(hipEventRecord(c.ev[3], st));
     FMM_HIP(hipStreamWaitEvent(st, c.evu[2], 0));  // join the upward pass
     FMM_HIP(hipEventRecord(c.ev[4], st));
@@ -1011,7 +1030,11 @@ static void hook_tree(fmm4rb_ctx *ctx, int64_t N, const double *pos, const doubl
     FMM_HIP(hipMemcpyAsync(ctx->h_mom.p, mom, bytes, hipMemcpyHostToDevice, st));
     prepare(c, N, N0, n, FMM4RB_FP64);
     stage_pavg(c, ctx->h_pos.as<double>(), ctx->h_mom.as<double>());
-    build_tree_device(c, ctx->h_pos.as<double>(), ctx->h_mom.as<double>(), N, N0);
+    build_tree_device(c, ctx->h_pos.as<double>(), ctx->h_mom.as<double>(), N, N0, false);
+    uint32_t tflag = 0;
+    FMM_HIP(hipMemcpyAsync(&tflag, c.d_tflag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipStreamSynchronize(st));
+    if (tflag) build_tree_device(c, ctx->h_pos.as<double>(), ctx->h_mom.as<double>(), N, N0, true);
     launch_node_geo(c);
 }
 
@@ -1055,7 +1078,7 @@ int fmm4rb_lists(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mo
     FMM_NEED(eta >= 0.0 && std::isfinite(eta), "eta must be finite and >= 0");
     hook_tree(ctx, N, pos, mom, N0, 1);
     Ctx &c = ctx->c;
-    build_lists_device(c, eta);
+    if (!build_lists_device(c, eta)) fail(FMM4RB_EDEVICE, "tree presort flag after the exact tree");
     const uint64_t mask = (1ull << c.sb) - 1ull;
     auto fetch = [&](const uint64_t *src, int64_t n, int64_t cap, int64_t *out) {
         const int64_t m = std::min(n, cap);

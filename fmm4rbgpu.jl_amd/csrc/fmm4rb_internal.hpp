@@ -125,18 +125,25 @@ struct Ctx;
 size_t radix_scratch_words(int64_t n, int nseg);
 bool radix_sort(uint64_t *keys, uint64_t *keys_alt, uint32_t *vals, uint32_t *vals_alt,
                 int64_t n, int nseg, int bit0, int bit1, uint32_t *scratch, hipStream_t st);
+bool radix_sort32(uint32_t *keys, uint32_t *keys_alt, uint32_t *vals, uint32_t *vals_alt,
+                  int64_t n, int nseg, int bit0, int bit1, uint32_t *scratch, hipStream_t st);
 // Exclusive scan of nrows rows of len words each; row totals to tot[nrows].
 void scan_rows(uint32_t *rows, int nrows, int len, uint32_t *tot, hipStream_t st);
 
 // ---- tree (tree.hip)
 void launch_pavg(const double *pos, const double *mom, int64_t N, double *part, double *pavg4,
                  int *bad, hipStream_t st);
-void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, int64_t N0);
+// exact = false: the presort uses 32-bit quantised keys and raises d_tflag if a run
+// of equal keys was too long to order (then build again with exact = true, 64-bit
+// keys of the exact coordinate order).
+void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, int64_t N0,
+                       bool exact);
 // New positions on the existing tree: tree-ordered particles and every bounding box.
 void refit_tree_device(Ctx &c, const double *pos, const double *mom);
 
 // ---- traversal (traverse.hip)
-void build_lists_device(Ctx &c, double eta);
+// Returns false, with no lists built, if the tree's quantised presort raised d_tflag.
+bool build_lists_device(Ctx &c, double eta);
 // Sort (target << sb | source) keys, CSR offsets per target; vals (may be null)
 // are permuted alongside, *vsorted = the buffer holding them sorted.
 void sort_pairs_csr(Ctx &c, uint64_t *keys, uint64_t *keys_alt, int64_t n, int32_t *off,
@@ -212,7 +219,7 @@ struct Ctx {
     DevBuf d_lo, d_cnt, d_flags, d_leaves, d_lvl_nodes, d_lvl_split, d_leftbase;
 
     // tree build scratch
-    DevBuf d_part, d_pavg, d_bad;
+    DevBuf d_part, d_pavg, d_bad, d_tflag;
     DevBuf d_keys, d_keys_alt, d_vals, d_vals_alt, d_rs_scratch;
     DevBuf d_seg, d_seg_alt, d_rk, d_pivot, d_bsum, d_axis;
     // tree outputs

@@ -1,4 +1,4 @@
-// radix_sort.hip -- stable LSD radix sort of 64-bit keys (with optional
+// radix_sort.hip -- stable LSD radix sort of 64-bit or 32-bit keys (with optional
 // 32-bit values) for gfx950, 8-bit digits, three kernels per pass:
 //   hist    : per-tile digit histogram (LDS atomics), digit-major table
 //   scan    : per-digit exclusive scan over tiles + digit totals
@@ -25,12 +25,13 @@ size_t radix_scratch_words(int64_t n, int nseg) {
     return (size_t)nseg * RS_BINS * (size_t)rs_nblk(n) + (size_t)nseg * RS_BINS;
 }
 
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restrict__ keys,
-                                                        int64_t n, int shift,
-                                                        uint32_t *__restrict__ hist, int nblk) {
+template <typename KT>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const KT *__restrict__ keys, int64_t n,
+                                                        int shift, uint32_t *__restrict__ hist,
+                                                        int nblk) {
     __shared__ uint32_t h[RS_BINS];
     const int seg = blockIdx.y;
-    const uint64_t *k = keys + (int64_t)seg * n;
+    const KT *k = keys + (int64_t)seg * n;
     h[threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
@@ -112,9 +113,9 @@ __global__ __launch_bounds__(ST) void k_scan_rows(uint32_t *__restrict__ rows, i
 // wave's own LDS accesses are in order, so no block barrier is needed between its
 // steps); one barrier and a per-digit prefix over the waves then give every key's
 // slot in the digit-ordered tile.
-template <bool HASVAL>
+template <typename KT, bool HASVAL>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
-    const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
+    const KT *__restrict__ kin, const uint32_t *__restrict__ vin, KT *__restrict__ kout,
     uint32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ hist, int nblk,
     const uint32_t *__restrict__ dtot) {
     constexpr int RS_WAVES = RS_THREADS / 64;
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     __shared__ uint32_t lst[RS_BINS];   // local position of the same key within the tile
     __shared__ uint32_t wcnt[RS_WAVES][RS_BINS];  // per-wave counts, then offsets over waves
     __shared__ uint32_t s[RS_THREADS];
-    __shared__ uint64_t sk[RS_TILE];
+    __shared__ KT sk[RS_TILE];
     __shared__ uint32_t sv[HASVAL ? RS_TILE : 1];
     const int seg = blockIdx.y;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -146,13 +147,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
     const int64_t wbase = base + (int64_t)wave * (RS_ITEMS * 64);
     const int tn = (int)min<int64_t>(RS_TILE, n - base);
-    uint64_t key[RS_ITEMS];
+    KT key[RS_ITEMS];
     uint32_t val[RS_ITEMS], rnk[RS_ITEMS];
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; ++r) {
         const int64_t i = wbase + r * 64 + lane;
         const bool valid = i < n;
-        key[r] = valid ? kin[so + i] : 0ull;
+        key[r] = valid ? kin[so + i] : (KT)0;
         val[r] = (HASVAL && valid) ? vin[so + i] : 0u;
     }
 #pragma unroll
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
     __syncthreads();
     // the tile in digit order: consecutive slots of one digit go to consecutive positions
     for (int j = tid; j < tn; j += RS_THREADS) {
-        const uint64_t k = sk[j];
+        const KT k = sk[j];
         const uint32_t dg = (uint32_t)(k >> shift) & 0xFFu;
         const uint32_t pos = doff[dg] + (uint32_t)j - lst[dg];
         kout[so + pos] = k;
@@ -216,31 +217,42 @@ void scan_rows(uint32_t *rows, int nrows, int len, uint32_t *tot, hipStream_t st
     scan_rows_grid(rows, dim3(nrows), len, tot, st);
 }
 
-bool radix_sort(uint64_t *keys, uint64_t *keys_alt, uint32_t *vals, uint32_t *vals_alt, int64_t n,
-                int nseg, int bit0, int bit1, uint32_t *scratch, hipStream_t st) {
+template <typename KT>
+static bool radix_sort_t(KT *keys, KT *keys_alt, uint32_t *vals, uint32_t *vals_alt, int64_t n,
+                         int nseg, int bit0, int bit1, uint32_t *scratch, hipStream_t st) {
     if (n <= 0 || bit1 <= bit0) return false;
     const int nblk = (int)rs_nblk(n);
     uint32_t *hist = scratch;
     uint32_t *dtot = scratch + (size_t)nseg * RS_BINS * nblk;
     bool alt = false;
     for (int shift = bit0; shift < bit1; shift += 8) {
-        uint64_t *ki = alt ? keys_alt : keys, *ko = alt ? keys : keys_alt;
+        KT *ki = alt ? keys_alt : keys, *ko = alt ? keys : keys_alt;
         uint32_t *vi = alt ? vals_alt : vals, *vo = alt ? vals : vals_alt;
         dim3 grid(nblk, nseg);
-        hipLaunchKernelGGL(k_rs_hist, grid, dim3(RS_THREADS), 0, st, ki, n, shift, hist, nblk);
+        hipLaunchKernelGGL(k_rs_hist<KT>, grid, dim3(RS_THREADS), 0, st, ki, n, shift, hist, nblk);
         FMM_LAUNCH_CHECK();
         scan_rows_grid(hist, dim3(RS_BINS, nseg), nblk, dtot, st);
         if (vals)
-            hipLaunchKernelGGL(k_rs_scatter<true>, grid, dim3(RS_THREADS), 0, st, ki, vi, ko, vo, n,
-                               shift, hist, nblk, dtot);
+            hipLaunchKernelGGL((k_rs_scatter<KT, true>), grid, dim3(RS_THREADS), 0, st, ki, vi, ko,
+                               vo, n, shift, hist, nblk, dtot);
         else
-            hipLaunchKernelGGL(k_rs_scatter<false>, grid, dim3(RS_THREADS), 0, st, ki,
+            hipLaunchKernelGGL((k_rs_scatter<KT, false>), grid, dim3(RS_THREADS), 0, st, ki,
                                (const uint32_t *)nullptr, ko, (uint32_t *)nullptr, n, shift, hist,
                                nblk, dtot);
         FMM_LAUNCH_CHECK();
         alt = !alt;
     }
     return alt;
+}
+
+bool radix_sort(uint64_t *keys, uint64_t *keys_alt, uint32_t *vals, uint32_t *vals_alt, int64_t n,
+                int nseg, int bit0, int bit1, uint32_t *scratch, hipStream_t st) {
+    return radix_sort_t(keys, keys_alt, vals, vals_alt, n, nseg, bit0, bit1, scratch, st);
+}
+
+bool radix_sort32(uint32_t *keys, uint32_t *keys_alt, uint32_t *vals, uint32_t *vals_alt,
+                  int64_t n, int nseg, int bit0, int bit1, uint32_t *scratch, hipStream_t st) {
+    return radix_sort_t(keys, keys_alt, vals, vals_alt, n, nseg, bit0, bit1, scratch, st);
 }
 
 }  // namespace fmm4rb

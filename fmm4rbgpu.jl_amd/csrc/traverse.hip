@@ -191,7 +191,7 @@ void transpose_lists(Ctx &c, const uint64_t *sorted, int64_t n, DevBuf &kT, DevB
     }
 }
 
-void build_lists_device(Ctx &c, double eta) {
+bool build_lists_device(Ctx &c, double eta) {
     const Shape &S = c.shape;
     hipStream_t st = c.stream;
     const int iters = 2 * S.ml + 1;
@@ -211,9 +211,11 @@ void build_lists_device(Ctx &c, double eta) {
                                c.d_pavg.as<double>(), eta);
             FMM_LAUNCH_CHECK();
         }
-        uint32_t h[2 * 29 + 4];  // ncnt <= 2 * maxlevel + 4, maxlevel <= 29
+        uint32_t h[2 * 29 + 4], tflag = 0;  // ncnt <= 2 * maxlevel + 4, maxlevel <= 29
         FMM_HIP(hipMemcpyAsync(h, cnt, sizeof(uint32_t) * ncnt, hipMemcpyDeviceToHost, st));
+        FMM_HIP(hipMemcpyAsync(&tflag, c.d_tflag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         FMM_HIP(hipStreamSynchronize(st));
+        if (tflag) return false;  // the tree is not the reference's: build it exactly
         uint64_t maxf = 0;
         for (int it = 0; it <= iters; ++it) maxf = std::max<uint64_t>(maxf, h[2 + it]);
         bool over = h[0] > (uint64_t)c.cap_p2p || h[1] > (uint64_t)c.cap_m2l ||
@@ -250,6 +252,7 @@ void build_lists_device(Ctx &c, double eta) {
                    c.d_p2p_off.as<int32_t>(), &c.p2p_sorted);
     sort_pairs_csr(c, c.d_m2l.as<uint64_t>(), c.d_m2l_alt.as<uint64_t>(), c.nm2l,
                    c.d_m2l_off.as<int32_t>(), &c.m2l_sorted);
+    return true;
 }
 
 }  // namespace fmm4rb

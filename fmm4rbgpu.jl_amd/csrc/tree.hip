@@ -48,12 +48,18 @@ __device__ inline uint64_t word_hash(uint64_t x, uint64_t idx) {
     return z ^ (z >> 31);
 }
 
+// The same pass also forms each axis' minimum and maximum position (the range of
+// the tree's quantised presort keys).
 __global__ __launch_bounds__(PAVG_THREADS) void k_pavg_partial(
     const double *__restrict__ pos, const double *__restrict__ mom, int64_t N,
-    double *__restrict__ part, unsigned long long *__restrict__ hpart, int *__restrict__ bad) {
+    double *__restrict__ part, unsigned long long *__restrict__ hpart, double *__restrict__ mmp,
+    int *__restrict__ bad) {
     __shared__ double s[3][PAVG_THREADS];
+    __shared__ double smm[6][PAVG_THREADS];
     __shared__ unsigned long long sh[PAVG_THREADS];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    double mn[3] = {__builtin_huge_val(), __builtin_huge_val(), __builtin_huge_val()};
+    double mx[3] = {-__builtin_huge_val(), -__builtin_huge_val(), -__builtin_huge_val()};
     uint64_t h = 0;
     int b = 0;
     for (int64_t i = (int64_t)blockIdx.x * PAVG_THREADS + threadIdx.x; i < N;
@@ -63,6 +69,12 @@ __global__ __launch_bounds__(PAVG_THREADS) void k_pavg_partial(
         a0 += m0;
         a1 += m1;
         a2 += m2;
+        mn[0] = fmin(mn[0], x0);
+        mn[1] = fmin(mn[1], x1);
+        mn[2] = fmin(mn[2], x2);
+        mx[0] = fmax(mx[0], x0);
+        mx[1] = fmax(mx[1], x1);
+        mx[2] = fmax(mx[2], x2);
         if (!(isfinite(m0) && isfinite(m1) && isfinite(m2) && isfinite(x0) && isfinite(x1) &&
               isfinite(x2)))
             b = 1;
@@ -73,38 +85,60 @@ __global__ __launch_bounds__(PAVG_THREADS) void k_pavg_partial(
     s[0][threadIdx.x] = a0;
     s[1][threadIdx.x] = a1;
     s[2][threadIdx.x] = a2;
+    for (int d = 0; d < 3; ++d) {
+        smm[d][threadIdx.x] = mn[d];
+        smm[3 + d][threadIdx.x] = mx[d];
+    }
     sh[threadIdx.x] = h;
     for (int off = PAVG_THREADS / 2; off > 0; off >>= 1) {
         __syncthreads();
         if (threadIdx.x < (unsigned)off) {
-            for (int d = 0; d < 3; ++d) s[d][threadIdx.x] += s[d][threadIdx.x + off];
+            for (int d = 0; d < 3; ++d) {
+                s[d][threadIdx.x] += s[d][threadIdx.x + off];
+                smm[d][threadIdx.x] = fmin(smm[d][threadIdx.x], smm[d][threadIdx.x + off]);
+                smm[3 + d][threadIdx.x] = fmax(smm[3 + d][threadIdx.x], smm[3 + d][threadIdx.x + off]);
+            }
             sh[threadIdx.x] += sh[threadIdx.x + off];
         }
     }
     if (threadIdx.x == 0) {
         for (int d = 0; d < 3; ++d) part[3 * blockIdx.x + d] = s[d][0];
+        for (int d = 0; d < 6; ++d) mmp[6 * blockIdx.x + d] = smm[d][0];
         hpart[blockIdx.x] = sh[0];
     }
     if (b) atomicOr(bad, 1);
 }
 
 __global__ __launch_bounds__(PAVG_THREADS) void k_pavg_final(
-    const double *__restrict__ part, const unsigned long long *__restrict__ hpart, int nblk,
-    int64_t N, double *__restrict__ pavg4, unsigned long long *__restrict__ digest) {
+    const double *__restrict__ part, const unsigned long long *__restrict__ hpart,
+    const double *__restrict__ mmp, int nblk, int64_t N, double *__restrict__ pavg4,
+    unsigned long long *__restrict__ digest) {
     __shared__ double s[3][PAVG_THREADS];
+    __shared__ double smm[6][PAVG_THREADS];
     __shared__ unsigned long long sh[PAVG_THREADS];
     double a[3] = {0.0, 0.0, 0.0};
+    double m[6] = {__builtin_huge_val(), __builtin_huge_val(), __builtin_huge_val(),
+                   -__builtin_huge_val(), -__builtin_huge_val(), -__builtin_huge_val()};
     uint64_t h = 0;
     for (int i = threadIdx.x; i < nblk; i += PAVG_THREADS) {
         for (int d = 0; d < 3; ++d) a[d] += part[3 * i + d];
+        for (int d = 0; d < 3; ++d) {
+            m[d] = fmin(m[d], mmp[6 * i + d]);
+            m[3 + d] = fmax(m[3 + d], mmp[6 * i + 3 + d]);
+        }
         h += hpart[i];
     }
     for (int d = 0; d < 3; ++d) s[d][threadIdx.x] = a[d];
+    for (int d = 0; d < 6; ++d) smm[d][threadIdx.x] = m[d];
     sh[threadIdx.x] = h;
     for (int off = PAVG_THREADS / 2; off > 0; off >>= 1) {
         __syncthreads();
         if (threadIdx.x < (unsigned)off) {
-            for (int d = 0; d < 3; ++d) s[d][threadIdx.x] += s[d][threadIdx.x + off];
+            for (int d = 0; d < 3; ++d) {
+                s[d][threadIdx.x] += s[d][threadIdx.x + off];
+                smm[d][threadIdx.x] = fmin(smm[d][threadIdx.x], smm[d][threadIdx.x + off]);
+                smm[3 + d][threadIdx.x] = fmax(smm[3 + d][threadIdx.x], smm[3 + d][threadIdx.x + off]);
+            }
             sh[threadIdx.x] += sh[threadIdx.x + off];
         }
     }
@@ -115,21 +149,25 @@ __global__ __launch_bounds__(PAVG_THREADS) void k_pavg_final(
         pavg4[2] = p2;
         pavg4[3] = sqrt(1.0 + ((p0 * p0 + p1 * p1) + p2 * p2));  // update_particles_field.jl:78
         digest[0] = sh[0];
+        // position range per axis: pavg4[8 + d] min, pavg4[11 + d] max
+        for (int d = 0; d < 6; ++d) pavg4[8 + d] = smm[d][0];
     }
 }
 
-// part: 3 PAVG_MAXBLK doubles then PAVG_MAXBLK digest words; pavg4: 4 doubles then
-// the digest word
+// part: 3 PAVG_MAXBLK doubles, PAVG_MAXBLK digest words, then 6 PAVG_MAXBLK doubles
+// (per-block position ranges); pavg4: p_avg, g_avg, the digest word, the flag word,
+// two spare words, then the position range (min x y z, max x y z)
 void launch_pavg(const double *pos, const double *mom, int64_t N, double *part, double *pavg4,
                  int *bad, hipStream_t st) {
     int nblk = (int)std::min<int64_t>(PAVG_MAXBLK, (N + PAVG_THREADS - 1) / PAVG_THREADS);
     unsigned long long *hpart = reinterpret_cast<unsigned long long *>(part + 3 * PAVG_MAXBLK);
+    double *mmp = part + 4 * PAVG_MAXBLK;
     unsigned long long *digest = reinterpret_cast<unsigned long long *>(pavg4 + 4);
     hipLaunchKernelGGL(k_pavg_partial, dim3(nblk), dim3(PAVG_THREADS), 0, st, pos, mom, N, part,
-                       hpart, bad);
+                       hpart, mmp, bad);
     FMM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_pavg_final, dim3(1), dim3(PAVG_THREADS), 0, st, part, hpart, nblk, N,
-                       pavg4, digest);
+    hipLaunchKernelGGL(k_pavg_final, dim3(1), dim3(PAVG_THREADS), 0, st, part, hpart, mmp, nblk,
+                       N, pavg4, digest);
     FMM_LAUNCH_CHECK();
 }
 
@@ -147,14 +185,97 @@ __device__ inline bool splitting(int32_t k, int l, const uint8_t *flags) {
 }
 
 __global__ void k_make_keys(const double *__restrict__ pos, int64_t N, uint64_t *__restrict__ keys,
-                            uint32_t *__restrict__ vals, int32_t *__restrict__ seg) {
+                            uint32_t *__restrict__ vals, int32_t *__restrict__ seg,
+                            uint32_t *__restrict__ flag) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
+    if (i == 0) *flag = 0u;
     for (int d = 0; d < 3; ++d) {
         keys[d * N + i] = ordered_key(pos[3 * i + d]);
         vals[d * N + i] = (uint32_t)i;
     }
     seg[i] = 0;
+}
+
+// Quantised presort: key = trunc((x - min_d) s_d) with s_d = QMAX / (max_d - min_d),
+// non-decreasing in x, so a stable sort of (key, id) from id order gives the (x, id)
+// order everywhere except inside runs of equal keys that hold different coordinates
+// (positions closer than the range / 2^32); k_presort_mark / k_presort_fix order
+// those runs.  4 radix passes over 4-byte keys instead of 8 over 8-byte ones.
+constexpr double PRESORT_QMAX = 4294967040.0;  // 2^32 - 256: the product's rounding stays below 2^32
+constexpr int PRESORT_RUN_CAP = 256;           // longest run the fix-up orders
+
+__global__ void k_make_keys32(const double *__restrict__ pos, int64_t N,
+                              const double *__restrict__ pavg4, uint32_t *__restrict__ keys,
+                              uint32_t *__restrict__ vals, int32_t *__restrict__ seg,
+                              uint8_t *__restrict__ need, uint32_t *__restrict__ flag) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (i == 0) *flag = 0u;
+    for (int d = 0; d < 3; ++d) {
+        const double mn = pavg4[8 + d], w = pavg4[11 + d] - mn;
+        const double q = PRESORT_QMAX / w;
+        const double sc = (w > 0.0 && q < 1e300) ? q : 0.0;
+        keys[d * N + i] = (uint32_t)((pos[3 * i + d] - mn) * sc);
+        vals[d * N + i] = (uint32_t)i;
+        need[d * N + i] = 0;
+    }
+    seg[i] = 0;
+}
+
+// A run of equal keys is out of (x, id) order exactly where a coordinate decreases
+// (ids ascend within the run): mark the run's first position.  A run longer than
+// PRESORT_RUN_CAP raises the flag instead (the caller then sorts the exact keys).
+__global__ void k_presort_mark(int64_t N, const uint32_t *__restrict__ keys,
+                               const uint32_t *__restrict__ perm, const double *__restrict__ pos,
+                               uint8_t *__restrict__ need, uint32_t *__restrict__ flag) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= 3 * N) return;
+    const int d = (int)(r / N);
+    const int64_t i = r - (int64_t)d * N;
+    const uint32_t *K = keys + (int64_t)d * N;
+    const uint32_t *P = perm + (int64_t)d * N;
+    if (i == 0 || K[i] != K[i - 1]) return;
+    const double xa = pos[3 * (int64_t)P[i - 1] + d] + 0.0, xb = pos[3 * (int64_t)P[i] + d] + 0.0;
+    if (!(xb < xa)) return;
+    int64_t h = i - 1;
+    while (h > 0 && K[h - 1] == K[i] && i - h < PRESORT_RUN_CAP) --h;
+    if (h > 0 && K[h - 1] == K[i]) {
+        atomicOr(flag, 1u);
+        return;
+    }
+    need[(int64_t)d * N + h] = 1;
+}
+
+// Insertion sort of each marked run by (x, id), by the thread of its first position.
+__global__ void k_presort_fix(int64_t N, const uint32_t *__restrict__ keys,
+                              uint32_t *__restrict__ perm, const double *__restrict__ pos,
+                              const uint8_t *__restrict__ need, uint32_t *__restrict__ flag) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= 3 * N || !need[r]) return;
+    const int d = (int)(r / N);
+    const int64_t h = r - (int64_t)d * N;
+    const uint32_t *K = keys + (int64_t)d * N;
+    uint32_t *P = perm + (int64_t)d * N;
+    int64_t e = h + 1;
+    while (e < N && K[e] == K[h] && e - h < PRESORT_RUN_CAP) ++e;
+    if (e < N && K[e] == K[h]) {
+        atomicOr(flag, 1u);
+        return;
+    }
+    for (int64_t a = h + 1; a < e; ++a) {
+        const uint32_t v = P[a];
+        const double xv = pos[3 * (int64_t)v + d] + 0.0;
+        int64_t b = a - 1;
+        while (b >= h) {
+            const uint32_t u = P[b];
+            const double xu = pos[3 * (int64_t)u + d] + 0.0;
+            if (xu < xv || (xu == xv && u < v)) break;
+            P[b + 1] = u;
+            --b;
+        }
+        P[b + 1] = v;
+    }
 }
 
 // rank4[p] = (r_x, r_y, r_z, -) with perm_d[r_d] = p: a particle's three ranks in
@@ -576,7 +697,8 @@ __global__ void k_refit_parents(int nnodes, const int32_t *__restrict__ nodes,
 // ------------------------------------------------------------- host side
 static inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
-void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, int64_t N0) {
+void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, int64_t N0,
+                       bool exact) {
     (void)N0;
     const Shape &S = c.shape;
     hipStream_t st = c.stream;
@@ -584,12 +706,34 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
     uint32_t *vals = c.d_vals.as<uint32_t>(), *vals_alt = c.d_vals_alt.as<uint32_t>();
     int32_t *seg = c.d_seg.as<int32_t>(), *seg_alt = c.d_seg_alt.as<int32_t>();
     uint32_t *rk = c.d_rk.as<uint32_t>(), *rk_alt = rk + 9 * (size_t)N;
-
-    hipLaunchKernelGGL(k_make_keys, dim3(nb(N, 256)), dim3(256), 0, st, pos, N, keys, vals, seg);
-    FMM_LAUNCH_CHECK();
-    bool alt = radix_sort(keys, keys_alt, vals, vals_alt, N, 3, 0, 64,
-                          c.d_rs_scratch.as<uint32_t>(), st);
-    const uint32_t *perm = alt ? vals_alt : vals;
+    uint32_t *flag = c.d_tflag.as<uint32_t>();
+    const uint32_t *perm;
+    if (exact) {  // 64-bit keys of the exact coordinate order
+        hipLaunchKernelGGL(k_make_keys, dim3(nb(N, 256)), dim3(256), 0, st, pos, N, keys, vals,
+                           seg, flag);
+        FMM_LAUNCH_CHECK();
+        const bool alt = radix_sort(keys, keys_alt, vals, vals_alt, N, 3, 0, 64,
+                                    c.d_rs_scratch.as<uint32_t>(), st);
+        perm = alt ? vals_alt : vals;
+    } else {
+        uint32_t *k32 = reinterpret_cast<uint32_t *>(keys);
+        uint32_t *k32a = reinterpret_cast<uint32_t *>(keys_alt);
+        uint8_t *need = reinterpret_cast<uint8_t *>(seg_alt);  // 3N bytes; the partition rewrites it
+        hipLaunchKernelGGL(k_make_keys32, dim3(nb(N, 256)), dim3(256), 0, st, pos, N,
+                           c.d_pavg.as<double>(), k32, vals, seg, need, flag);
+        FMM_LAUNCH_CHECK();
+        const bool alt = radix_sort32(k32, k32a, vals, vals_alt, N, 3, 0, 32,
+                                      c.d_rs_scratch.as<uint32_t>(), st);
+        const uint32_t *ks = alt ? k32a : k32;
+        uint32_t *pm = alt ? vals_alt : vals;
+        hipLaunchKernelGGL(k_presort_mark, dim3(nb(3 * N, 256)), dim3(256), 0, st, N, ks, pm, pos,
+                           need, flag);
+        FMM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_presort_fix, dim3(nb(3 * N, 256)), dim3(256), 0, st, N, ks, pm, pos,
+                           need, flag);
+        FMM_LAUNCH_CHECK();
+        perm = pm;
+    }
     // the sorted keys are not needed any more: their buffer (24 B per particle) holds
     // the 16-byte rank words
     uint32_t *rank4 = reinterpret_cast<uint32_t *>(keys);

@@ -38,11 +38,25 @@ def _case(name):
         return W.bunch(300, seed=6), 1
     if name == "n0_1024":  # leaves of up to 1024 particles: several target passes per block
         return W.bunch(20000, seed=7), 1024
+    if name == "near_ties":
+        # copies of 512 particles moved by +-1e-13 per axis, below the presort's key
+        # resolution (range / 2^32 = 2.3e-10): runs of equal keys with different
+        # coordinates, half of them in descending id order, which the fix-up orders
+        pos, mom = W.uniform_cube(4096, seed=8)
+        sgn = np.random.default_rng(9).choice([-1.0, 1.0], size=(512, 3))
+        pos[3584:] = pos[:512] + sgn * 1e-13
+        return (pos, mom), 125
+    if name == "outlier":
+        # one particle 1e12 away on x: every other x shares one presort key (a run
+        # longer than the fix-up takes), so the tree is built again from exact keys
+        pos, mom = W.uniform_cube(4096, seed=10)
+        pos[17, 0] = 1e12
+        return (pos, mom), 125
     raise KeyError(name)
 
 
 CASES = ["uniform8192", "bunch32k", "ties", "earlyleaf", "single", "small_n0", "n0_1",
-         "n0_1024"]
+         "n0_1024", "near_ties", "outlier"]
 
 
 def _oracle_tree(orc, pos, N0, pavg):
@@ -65,6 +79,7 @@ def test_tree_bit_exact(ctx, orc, name):
 
 @pytest.mark.parametrize("name,eta", [("uniform8192", 0.5), ("bunch32k", 0.5), ("ties", 0.7),
                                       ("earlyleaf", 0.5), ("single", 0.5), ("small_n0", 0.3),
+                                      ("near_ties", 0.5), ("outlier", 0.5),
                                       ("uniform8192", 0.0), ("bunch32k", 999.0)])
 def test_lists_exact(ctx, orc, name, eta):
     (pos, mom), N0 = _case(name)
@@ -130,7 +145,7 @@ def test_fp64_fmm_uniform8192(ctx, orc, n):
 
 
 @pytest.mark.parametrize("name", ["bunch32k", "earlyleaf", "single", "small_n0", "ties", "n0_1",
-                                  "n0_1024"])
+                                  "n0_1024", "near_ties", "outlier"])
 def test_fp64_fmm_cases(ctx, orc, name):
     (pos, mom), N0 = _case(name)
     E, B, st = ctx.field(pos, mom, -1.0, 1e-3, 4, N0, 0.5, "fp64")
