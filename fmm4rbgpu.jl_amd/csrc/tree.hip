@@ -461,6 +461,127 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(
     }
 }
 
+// ------------------------------------------------ middle levels, per cluster
+// Once every cluster of a level holds at most PL_MAXC particles, one workgroup per
+// (cluster, list) partitions the cluster's segment of that list by itself: it
+// walks the segment in chunks of PL_CHUNK entries in order, carrying the count of
+// left entries, so neither the per-block count pass and its global scan nor the
+// segment ids are needed; the workgroup also forms its cluster's bounding box, axis
+// and pivot (as k_level_nodes; the three workgroups of a cluster compute the same
+// values from the same entries, and the list-0 one stores them).  Entries of a
+// leaf are copied unchanged.
+constexpr int PL_I = 8, PL_MAXC = 1 << 16;
+
+// PL_T threads (1024, or 256 for clusters of at most 4096 particles, whose one or two
+// chunks would leave most of a larger block idle).  The first chunk's entries are
+// loaded before the cluster's split is known, so those loads overlap the dependent
+// gathers of the bounding box (six lanes, one per axis and end).
+template <int PL_T>
+__global__ __launch_bounds__(PL_T) void k_part_local(
+    const int32_t *__restrict__ nodes, const int32_t *__restrict__ lo,
+    const int32_t *__restrict__ cnt, const uint8_t *__restrict__ flags,
+    const uint32_t *__restrict__ perm, int64_t N, const double *__restrict__ pos,
+    const double *__restrict__ pavg4, const uint32_t *__restrict__ rk,
+    uint32_t *__restrict__ rk_out, double *__restrict__ bbox, int8_t *__restrict__ axis,
+    int32_t *__restrict__ ext) {
+    constexpr int NW = PL_T / 64, PL_CHUNK = PL_T * PL_I;
+    __shared__ uint32_t s[2][PL_I][NW];
+    __shared__ double sx[6];
+    __shared__ int32_t sp[6];
+    const int d = blockIdx.y, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int32_t k = nodes[blockIdx.x];
+    const int64_t b = lo[k];
+    const int c = cnt[k];
+    const bool split = (flags[k] & 3) == 1;
+    const uint32_t *in = rk + (int64_t)(3 * d) * N;
+    uint32_t *out = rk_out + (int64_t)(3 * d) * N;
+    uint32_t v[PL_I][3];  // the current chunk's entries
+    auto load = [&](int base) {
+#pragma unroll
+        for (int j = 0; j < PL_I; ++j) {
+            const int i = base + j * PL_T + tid;
+#pragma unroll
+            for (int e = 0; e < 3; ++e) v[j][e] = i < c ? in[e * N + b + i] : 0u;
+        }
+    };
+    load(0);
+    if (tid < 6) {  // clusters.jl:51-70: lane 2 e + end, the particle at list e's end
+        const int e = tid >> 1;
+        const int64_t at = (tid & 1) ? b + c - 1 : b;
+        const int32_t p = (int32_t)perm[e * N + rk[(4 * e) * N + at]];  // list e, own rank
+        sp[tid] = p;
+        sx[tid] = pos[3 * (int64_t)p + e] + 0.0;
+    }
+    __syncthreads();
+    int dir = 0;  // clusters.jl:39, the same in every lane
+    {
+        const double sc[3] = {1.0, 1.0, pavg4[3]};
+        double best = sc[0] * (sx[1] - sx[0]);
+        for (int e = 1; e < 3; ++e) {
+            const double q = sc[e] * (sx[2 * e + 1] - sx[2 * e]);
+            if (q > best) {
+                best = q;
+                dir = e;
+            }
+        }
+    }
+    if (d == 0 && tid < 6) {
+        const int e = tid >> 1, hi = tid & 1;
+        bbox[6 * (int64_t)k + 3 * hi + e] = sx[tid];
+        ext[6 * (int64_t)k + 3 * hi + e] = sp[tid];
+        if (tid == 0 && split) axis[k] = (int8_t)dir;
+    }
+    if (!split) {
+        for (int base = 0; base < c; base += PL_CHUNK) {
+            if (base > 0) load(base);
+#pragma unroll
+            for (int j = 0; j < PL_I; ++j) {
+                const int i = base + j * PL_T + tid;
+                if (i < c)
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) out[e * N + b + i] = v[j][e];
+            }
+        }
+        return;
+    }
+    const uint32_t pv = rk[(4 * dir) * N + b + ((c + 1) >> 1)];  // first right entry of list dir
+    const int64_t nl = (c + 1) >> 1;
+    const uint64_t below = (1ull << lane) - 1ull;
+    int64_t carry = 0;  // left entries before the chunk
+    for (int base = 0, ph = 0; base < c; base += PL_CHUNK, ph ^= 1) {
+        if (base > 0) load(base);
+        uint64_t bal[PL_I];
+#pragma unroll
+        for (int j = 0; j < PL_I; ++j) {
+            const int i = base + j * PL_T + tid;
+            const uint32_t ra = dir == 0 ? v[j][0] : (dir == 1 ? v[j][1] : v[j][2]);
+            bal[j] = __ballot(i < c && ra < pv);
+            if (lane == 0) s[ph][j][w] = (uint32_t)__popcll(bal[j]);
+        }
+        __syncthreads();  // s[ph] complete (s[ph ^ 1] was last read before the previous barrier)
+        int64_t run = carry;
+#pragma unroll
+        for (int j = 0; j < PL_I; ++j) {
+            uint32_t before = 0, total = 0;
+#pragma unroll
+            for (int u = 0; u < NW; ++u) {
+                const uint32_t t = s[ph][j][u];
+                before += u < w ? t : 0u;
+                total += t;
+            }
+            const int i = base + j * PL_T + tid;
+            if (i < c) {
+                const int64_t lft = run + before + (uint32_t)__popcll(bal[j] & below);
+                const int64_t np = ((bal[j] >> lane) & 1ull) ? lft : nl + i - lft;
+#pragma unroll
+                for (int e = 0; e < 3; ++e) out[e * N + b + np] = v[j][e];
+            }
+            run += total;
+        }
+        carry = run;
+    }
+}
+
 // ------------------------------------------------------- deep levels in LDS
 // Once every cluster of a level L0 holds at most TF_CAP particles, one workgroup
 // per cluster finishes its subtree in LDS: for each level l = L0 .. ml the
@@ -759,7 +880,18 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
             break;
         }
     }
+    // first level whose clusters all fit one workgroup's chunked pass (k_part_local)
+    int L1 = L0;
     for (int l = 0; l < L0; ++l) {
+        int mx = 0;
+        for (int64_t q = S.lvl_off[l]; q < S.lvl_off[l + 1]; ++q)
+            mx = std::max(mx, S.cnt[S.lvl_nodes[q]]);
+        if (mx <= PL_MAXC) {
+            L1 = l;
+            break;
+        }
+    }
+    for (int l = 0; l < L1; ++l) {
         int nnodes = (int)(S.lvl_off[l + 1] - S.lvl_off[l]);
         if (nnodes > 0) {
             hipLaunchKernelGGL(k_level_nodes, dim3(nb(nnodes, 128)), dim3(128), 0, st, nnodes,
@@ -780,6 +912,25 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
         FMM_LAUNCH_CHECK();
         std::swap(rk, rk_alt);
         std::swap(seg, seg_alt);
+    }
+    for (int l = L1; l < L0; ++l) {
+        const int nnodes = (int)(S.lvl_off[l + 1] - S.lvl_off[l]);
+        if (nnodes == 0) continue;
+        int mx = 0;
+        for (int64_t q = S.lvl_off[l]; q < S.lvl_off[l + 1]; ++q)
+            mx = std::max(mx, S.cnt[S.lvl_nodes[q]]);
+        if (mx <= 2 * 256 * PL_I)
+            hipLaunchKernelGGL(k_part_local<256>, dim3(nnodes, 3), dim3(256), 0, st,
+                               c.d_lvl_nodes.as<int32_t>() + S.lvl_off[l], lo, cnt, flags, perm,
+                               N, pos, c.d_pavg.as<double>(), rk, rk_alt, c.d_bbox.as<double>(),
+                               axis, c.d_ext.as<int32_t>());
+        else
+            hipLaunchKernelGGL(k_part_local<1024>, dim3(nnodes, 3), dim3(1024), 0, st,
+                               c.d_lvl_nodes.as<int32_t>() + S.lvl_off[l], lo, cnt, flags, perm,
+                               N, pos, c.d_pavg.as<double>(), rk, rk_alt, c.d_bbox.as<double>(),
+                               axis, c.d_ext.as<int32_t>());
+        FMM_LAUNCH_CHECK();
+        std::swap(rk, rk_alt);
     }
     {
         const int nnodes = (int)(S.lvl_off[L0 + 1] - S.lvl_off[L0]);
