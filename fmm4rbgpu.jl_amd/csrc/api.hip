@@ -102,13 +102,13 @@ static void *scratch_ptr(int device) {
 constexpr int CTX_NBUF_MAX = 80;
 static int ctx_bufs(Ctx &c, DevBuf **out) {
     DevBuf *all[] = {&c.d_lo, &c.d_cnt, &c.d_flags, &c.d_leaves, &c.d_lvl_nodes, &c.d_lvl_split,
-            &c.d_leftbase, &c.d_part, &c.d_pavg, &c.d_bad, &c.d_tflag, &c.d_keys, &c.d_keys_alt,
+            &c.d_leftbase, &c.d_part, &c.d_pavg, &c.d_bad, &c.d_keys, &c.d_keys_alt,
             &c.d_vals, &c.d_vals_alt, &c.d_rs_scratch, &c.d_seg, &c.d_seg_alt,
             &c.d_rk, &c.d_pivot, &c.d_bsum, &c.d_axis, &c.d_parind, &c.d_leafof,
             &c.d_bbox, &c.d_geo, &c.d_ncoord, &c.d_costab, &c.d_ext, &c.d_sp8, &c.d_sp8f,
             &c.d_ob, &c.d_gx, &c.d_gp, &c.d_gx2, &c.d_gp2, &c.d_xbar2, &c.d_lbar, &c.d_mbar,
             &c.d_xbar, &c.d_pbar, &c.d_abar, &c.d_p2pT, &c.d_p2pT_alt, &c.d_m2lT, &c.d_m2lT_alt,
-            &c.d_p2pT_off, &c.d_m2lT_off, &c.d_front0, &c.d_front1, &c.d_tcnt, &c.d_p2p,
+            &c.d_p2pT_off, &c.d_m2lT_off, &c.d_front0, &c.d_front1, &c.d_tcnt, &c.d_pairs, &c.d_pairs_alt, &c.d_p2p,
             &c.d_p2p_alt, &c.d_m2l, &c.d_m2l_alt, &c.d_p2p_off, &c.d_m2l_off, &c.d_mp4,
             &c.d_mp4f, &c.d_loc, &c.d_near, &c.d_m2lT_pos, &c.d_ysp};
     constexpr int nb = (int)(sizeof(all) / sizeof(all[0]));
@@ -246,7 +246,6 @@ static void prepare(Ctx &c, int64_t N, int64_t N0, int n, int precision) {
     // p_avg, g_avg, input digest, the non-finite flag, 2 spare, position range
     c.d_pavg.ensure(16 * sizeof(double));
     c.d_bad.ensure(sizeof(int));
-    c.d_tflag.ensure(sizeof(uint32_t));
     c.d_keys.ensure(n3 * 8);
     c.d_keys_alt.ensure(n3 * 8);
     c.d_vals.ensure(n3 * 4);
@@ -270,17 +269,17 @@ static void prepare(Ctx &c, int64_t N, int64_t N0, int n, int precision) {
     c.cap_front = std::max<int64_t>(c.cap_front, std::max<int64_t>(1 << 16, 48 * nleaf));
     c.cap_p2p = std::max<int64_t>(c.cap_p2p, std::max<int64_t>(1 << 16, 64 * nleaf));
     c.cap_m2l = std::max<int64_t>(c.cap_m2l, std::max<int64_t>(1 << 16, 64 * nleaf));
-    // the sort scratch also covers the pair sorts of lists up to their capacity
+    // the sort scratch also covers the joint sort of the two lists up to their capacity
     c.d_rs_scratch.ensure(
         4 * std::max(radix_scratch_words(N, 3),
-                     radix_scratch_words(std::max(c.cap_p2p, c.cap_m2l), 1)));
+                     radix_scratch_words(std::max(c.cap_p2p, c.cap_m2l), 2)));
     c.d_front0.ensure(8 * (size_t)c.cap_front);
     c.d_front1.ensure(8 * (size_t)c.cap_front);
-    c.d_tcnt.ensure(4 * (size_t)(2 * S.ml + 8));
+    c.d_tcnt.ensure(4 * (size_t)TCNT_WORDS);
     c.d_p2p.ensure(8 * (size_t)c.cap_p2p);
-    c.d_p2p_alt.ensure(8 * (size_t)c.cap_p2p);
     c.d_m2l.ensure(8 * (size_t)c.cap_m2l);
-    c.d_m2l_alt.ensure(8 * (size_t)c.cap_m2l);
+    c.d_pairs.ensure(2 * 8 * (size_t)std::max(c.cap_p2p, c.cap_m2l));
+    c.d_pairs_alt.ensure(2 * 8 * (size_t)std::max(c.cap_p2p, c.cap_m2l));
     c.d_p2p_off.ensure(4 * (nc + 1));
     c.d_m2l_off.ensure(4 * (nc + 1));
     c.d_mp4.ensure(4 * NN * nc * 8);
@@ -412,6 +411,23 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
     FMM_HIP(hipEventRecord(c.evx[1], sn));
     launch_p2p(c, c.d_p2p_off.as<int32_t>(), c.p2p_sorted, c.sb, sn);
     FMM_HIP(hipEventRecord(c.evx[2], sn));
+    // list statistics for the caller, also on the near-field stream (off the far
+    // field's path)
+    unsigned long long ls[2] = {0, 0};
+    if (stats) {
+        DevBuf &tmp = c.d_bad;  // reuse: needs 16 bytes
+        tmp.ensure(16);
+        zero_words(tmp.p, 4, sn);
+        const int64_t mx = std::max(c.np2p, c.nm2l);
+        if (mx > 0) {
+            hipLaunchKernelGGL(k_list_stats, dim3((unsigned)std::min<int64_t>(1024, (mx + 255) / 256)),
+                               dim3(256), 0, sn,
+                               c.p2p_sorted, c.np2p, c.m2l_sorted, c.nm2l, c.sb,
+                               c.d_cnt.as<int32_t>(), tmp.as<unsigned long long>());
+            FMM_LAUNCH_CHECK();
+        }
+        FMM_HIP(hipMemcpyAsync(ls, tmp.p, 16, hipMemcpyDeviceToHost, sn));
+    }
     FMM_HIP(hipStreamWaitEvent(st, c.evu[2], 0));  // join the upward pass
     FMM_HIP(hipEventRecord(c.ev[4], st));
     launch_m2l(c, c.d_m2l_off.as<int32_t>(), c.m2l_sorted, c.sb, c.shape.nc);
@@ -427,22 +443,8 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
     FMM_HIP(hipEventRecord(c.ev[6], st));
     launch_leaf_l2p(c, amp, E, B);
     FMM_HIP(hipEventRecord(c.ev[7], st));
-    unsigned long long ls[2] = {0, 0};
-    if (stats) {
-        DevBuf &tmp = c.d_bad;  // reuse: needs 16 bytes
-        tmp.ensure(16);
-        zero_words(tmp.p, 4, st);
-        const int64_t mx = std::max(c.np2p, c.nm2l);
-        if (mx > 0) {
-            hipLaunchKernelGGL(k_list_stats, dim3((unsigned)std::min<int64_t>(1024, (mx + 255) / 256)),
-                               dim3(256), 0, st,
-                               c.p2p_sorted, c.np2p, c.m2l_sorted, c.nm2l, c.sb,
-                               c.d_cnt.as<int32_t>(), tmp.as<unsigned long long>());
-            FMM_LAUNCH_CHECK();
-        }
-        FMM_HIP(hipMemcpyAsync(ls, tmp.p, 16, hipMemcpyDeviceToHost, st));
-    }
     FMM_HIP(hipStreamSynchronize(st));
+    if (stats) FMM_HIP(hipStreamSynchronize(sn));
     c.tree_valid = true;
     c.tree_N = N;
     c.tree_N0 = N0;
@@ -1032,7 +1034,8 @@ static void hook_tree(fmm4rb_ctx *ctx, int64_t N, const double *pos, const doubl
     stage_pavg(c, ctx->h_pos.as<double>(), ctx->h_mom.as<double>());
     build_tree_device(c, ctx->h_pos.as<double>(), ctx->h_mom.as<double>(), N, N0, false);
     uint32_t tflag = 0;
-    FMM_HIP(hipMemcpyAsync(&tflag, c.d_tflag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(&tflag, c.d_tcnt.as<uint32_t>() + TCNT_FLAG, sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, st));
     FMM_HIP(hipStreamSynchronize(st));
     if (tflag) build_tree_device(c, ctx->h_pos.as<double>(), ctx->h_mom.as<double>(), N, N0, true);
     launch_node_geo(c);

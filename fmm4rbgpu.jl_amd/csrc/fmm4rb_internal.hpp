@@ -82,6 +82,7 @@ __device__ inline T rsq3(T d) {
     return (y * y) * y;
 }
 constexpr int MAX_N0 = 1024;
+constexpr int TCNT_WORDS = 64, TCNT_FLAG = 63;  // 2 maxlevel + 4 <= 62 counters, then the flag
 
 // Workgroup b of a launch runs on XCD b % 8 (round-robin dispatch over the
 // MI355X's 8 XCDs, each with its own L2).  The kernels that walk interaction
@@ -133,16 +134,16 @@ void scan_rows(uint32_t *rows, int nrows, int len, uint32_t *tot, hipStream_t st
 // ---- tree (tree.hip)
 void launch_pavg(const double *pos, const double *mom, int64_t N, double *part, double *pavg4,
                  int *bad, hipStream_t st);
-// exact = false: the presort uses 32-bit quantised keys and raises d_tflag if a run
-// of equal keys was too long to order (then build again with exact = true, 64-bit
-// keys of the exact coordinate order).
+// exact = false: the presort uses 32-bit quantised keys and raises the tree flag
+// (word TCNT_FLAG of d_tcnt) if a run of equal keys was too long to order (then build
+// again with exact = true, 64-bit keys of the exact coordinate order).
 void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, int64_t N0,
                        bool exact);
 // New positions on the existing tree: tree-ordered particles and every bounding box.
 void refit_tree_device(Ctx &c, const double *pos, const double *mom);
 
 // ---- traversal (traverse.hip)
-// Returns false, with no lists built, if the tree's quantised presort raised d_tflag.
+// Returns false, with no lists built, if the tree's quantised presort raised its flag.
 bool build_lists_device(Ctx &c, double eta);
 // Sort (target << sb | source) keys, CSR offsets per target; vals (may be null)
 // are permuted alongside, *vsorted = the buffer holding them sorted.
@@ -219,14 +220,19 @@ struct Ctx {
     DevBuf d_lo, d_cnt, d_flags, d_leaves, d_lvl_nodes, d_lvl_split, d_leftbase;
 
     // tree build scratch
-    DevBuf d_part, d_pavg, d_bad, d_tflag;
+    DevBuf d_part, d_pavg, d_bad;
     DevBuf d_keys, d_keys_alt, d_vals, d_vals_alt, d_rs_scratch;
     DevBuf d_seg, d_seg_alt, d_rk, d_pivot, d_bsum, d_axis;
     // tree outputs
     DevBuf d_parind, d_leafof, d_bbox, d_geo, d_ncoord, d_costab, d_ext;
     DevBuf d_sp8, d_sp8f;
     // lists
+    // d_tcnt: the traversal's counters (words 0 .. 2 ml + 3) and the tree flag (word
+    // TCNT_FLAG), read back in one copy
     DevBuf d_front0, d_front1, d_tcnt, d_p2p, d_p2p_alt, d_m2l, d_m2l_alt, d_p2p_off, d_m2l_off;
+    // both lists side by side (stride pair_stride) for their joint sort
+    DevBuf d_pairs, d_pairs_alt;
+    int64_t pair_stride = 0;
     int64_t cap_front = 0, cap_p2p = 0, cap_m2l = 0;
     int64_t np2p = 0, nm2l = 0;
     int sb = 1;

@@ -44,73 +44,100 @@ __global__ void k_trav_init(int2 *front, uint32_t *cnt, int ncnt) {
     if (i == 0) cnt[2] = 1;
 }
 
+// TR_ITEMS pairs per thread per step (independent loads in flight, one atomic per
+// category per TR_ITEMS x TR_THREADS pairs); the outputs of a step are placed in
+// (item, wave, lane) order after the workgroup's reservation.
+constexpr int TR_ITEMS = 4;
+
 __global__ __launch_bounds__(TR_THREADS) void k_traverse(
     const int2 *__restrict__ fin, int2 *__restrict__ fout, uint32_t *__restrict__ cnt, int it,
     uint32_t cap_f, uint64_t *__restrict__ p2p, uint32_t cap_p2p, uint64_t *__restrict__ m2l,
     uint32_t cap_m2l, int sb, const double *__restrict__ geo, const uint8_t *__restrict__ flags,
     const double *__restrict__ pavg4, double eta) {
     constexpr int NW = TR_THREADS / 64;
-    __shared__ uint32_t wc[3][NW], bb[3];
+    __shared__ uint32_t wc[3][TR_ITEMS][NW], bb[3];
     const uint32_t n = min(cnt[2 + it], cap_f);
     const double g = pavg4[3];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (uint32_t base = blockIdx.x * TR_THREADS; base < n; base += gridDim.x * TR_THREADS) {
-        const uint32_t i = base + threadIdx.x;
-        int cat = -1, t = 0, s = 0;
-        if (i < n) {
-            int2 pr = fin[i];
-            t = pr.x;
-            s = pr.y;
-            const bool tl = flags[t] & 2, sl = flags[s] & 2;
-            const double *gt = geo + 8 * (int64_t)t, *gs = geo + 8 * (int64_t)s;
-            if (tl && sl) cat = 0;
-            else if (admissible(gt, gs, g, eta)) cat = 1;
-            else if (tl) cat = 3;
-            else if (sl) cat = 2;
-            else cat = gt[4] > gs[4] ? 2 : 3;  // diameter(t) > diameter(s)
+    constexpr uint32_t STEP = TR_ITEMS * TR_THREADS;
+    for (uint32_t base = blockIdx.x * STEP; base < n; base += gridDim.x * STEP) {
+        int cat[TR_ITEMS], t[TR_ITEMS], s[TR_ITEMS];
+        int2 pr[TR_ITEMS];
+#pragma unroll
+        for (int j = 0; j < TR_ITEMS; ++j) {
+            const uint32_t i = base + j * TR_THREADS + threadIdx.x;
+            pr[j] = i < n ? fin[i] : make_int2(-1, -1);
         }
-        const uint64_t m0 = __ballot(cat == 0), m1 = __ballot(cat == 1), m2 = __ballot(cat >= 2);
-        // one atomic per category per workgroup; waves take consecutive ranges
-        if (lane == 0) {
-            wc[0][wave] = (uint32_t)__popcll(m0);
-            wc[1][wave] = (uint32_t)__popcll(m1);
-            wc[2][wave] = 2u * (uint32_t)__popcll(m2);
+#pragma unroll
+        for (int j = 0; j < TR_ITEMS; ++j) {
+            t[j] = pr[j].x;
+            s[j] = pr[j].y;
+            cat[j] = -1;
+            if (t[j] >= 0) {
+                const bool tl = flags[t[j]] & 2, sl = flags[s[j]] & 2;
+                const double *gt = geo + 8 * (int64_t)t[j], *gs = geo + 8 * (int64_t)s[j];
+                if (tl && sl) cat[j] = 0;
+                else if (admissible(gt, gs, g, eta)) cat[j] = 1;
+                else if (tl) cat[j] = 3;
+                else if (sl) cat[j] = 2;
+                else cat[j] = gt[4] > gs[4] ? 2 : 3;  // diameter(t) > diameter(s)
+            }
+        }
+        uint64_t m[3][TR_ITEMS];
+#pragma unroll
+        for (int j = 0; j < TR_ITEMS; ++j) {
+            m[0][j] = __ballot(cat[j] == 0);
+            m[1][j] = __ballot(cat[j] == 1);
+            m[2][j] = __ballot(cat[j] >= 2);
+            if (lane == 0) {
+                wc[0][j][wave] = (uint32_t)__popcll(m[0][j]);
+                wc[1][j][wave] = (uint32_t)__popcll(m[1][j]);
+                wc[2][j][wave] = 2u * (uint32_t)__popcll(m[2][j]);
+            }
         }
         __syncthreads();
-        if (threadIdx.x < 3) {
+        if (threadIdx.x < 3) {  // one atomic per category per step
             const int k = threadIdx.x;
             uint32_t tot = 0;
-            for (int w = 0; w < NW; ++w) tot += wc[k][w];
+            for (int j = 0; j < TR_ITEMS; ++j)
+                for (int w = 0; w < NW; ++w) tot += wc[k][j][w];
             uint32_t *ctr = k == 2 ? &cnt[2 + it + 1] : &cnt[k];
             bb[k] = tot ? atomicAdd(ctr, tot) : 0u;
         }
         __syncthreads();
-        uint32_t b0 = bb[0], b1 = bb[1], b2 = bb[2];
-        for (int w = 0; w < wave; ++w) {
-            b0 += wc[0][w];
-            b1 += wc[1][w];
-            b2 += wc[2][w];
-        }
-        __syncthreads();  // wc and bb are rewritten in the next chunk
-        if (cat == 0) {
-            uint32_t o = b0 + (uint32_t)__popcll(m0 & lt);
-            if (o < cap_p2p) p2p[o] = ((uint64_t)t << sb) | (uint64_t)s;
-        } else if (cat == 1) {
-            uint32_t o = b1 + (uint32_t)__popcll(m1 & lt);
-            if (o < cap_m2l) m2l[o] = ((uint64_t)t << sb) | (uint64_t)s;
-        } else if (cat >= 2) {
-            uint32_t o = b2 + 2u * (uint32_t)__popcll(m2 & lt);
-            if (o + 1 < cap_f) {
-                if (cat == 2) {
-                    fout[o] = make_int2(2 * t + 1, s);
-                    fout[o + 1] = make_int2(2 * t + 2, s);
-                } else {
-                    fout[o] = make_int2(t, 2 * s + 1);
-                    fout[o + 1] = make_int2(t, 2 * s + 2);
+        uint32_t o[3] = {bb[0], bb[1], bb[2]};
+#pragma unroll
+        for (int j = 0; j < TR_ITEMS; ++j) {
+            uint32_t pre[3] = {0u, 0u, 0u}, tot[3] = {0u, 0u, 0u};
+            for (int w = 0; w < NW; ++w)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    pre[k] += w < wave ? wc[k][j][w] : 0u;
+                    tot[k] += wc[k][j][w];
+                }
+            if (cat[j] == 0) {
+                const uint32_t q = o[0] + pre[0] + (uint32_t)__popcll(m[0][j] & lt);
+                if (q < cap_p2p) p2p[q] = ((uint64_t)t[j] << sb) | (uint64_t)s[j];
+            } else if (cat[j] == 1) {
+                const uint32_t q = o[1] + pre[1] + (uint32_t)__popcll(m[1][j] & lt);
+                if (q < cap_m2l) m2l[q] = ((uint64_t)t[j] << sb) | (uint64_t)s[j];
+            } else if (cat[j] >= 2) {
+                const uint32_t q = o[2] + pre[2] + 2u * (uint32_t)__popcll(m[2][j] & lt);
+                if (q + 1 < cap_f) {
+                    if (cat[j] == 2) {
+                        fout[q] = make_int2(2 * t[j] + 1, s[j]);
+                        fout[q + 1] = make_int2(2 * t[j] + 2, s[j]);
+                    } else {
+                        fout[q] = make_int2(t[j], 2 * s[j] + 1);
+                        fout[q + 1] = make_int2(t[j], 2 * s[j] + 2);
+                    }
                 }
             }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) o[k] += tot[k];
         }
+        __syncthreads();  // wc and bb are rewritten in the next step
     }
 }
 
@@ -127,6 +154,16 @@ __global__ void k_csr(const uint64_t *__restrict__ keys, int64_t n, int sb, int6
         else b = m;
     }
     off[t] = (int32_t)a;
+}
+
+// The two lists side by side, each padded to L keys with pad (above every pair key).
+__global__ void k_pack_pairs(const uint64_t *__restrict__ p2p, int64_t np2p,
+                             const uint64_t *__restrict__ m2l, int64_t nm2l, int64_t L,
+                             uint64_t pad, uint64_t *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L) return;
+    out[i] = i < np2p ? p2p[i] : pad;
+    out[L + i] = i < nm2l ? m2l[i] : pad;
 }
 
 void sort_pairs_csr(Ctx &c, uint64_t *keys, uint64_t *keys_alt, int64_t n, int32_t *off,
@@ -211,11 +248,10 @@ bool build_lists_device(Ctx &c, double eta) {
                                c.d_pavg.as<double>(), eta);
             FMM_LAUNCH_CHECK();
         }
-        uint32_t h[2 * 29 + 4], tflag = 0;  // ncnt <= 2 * maxlevel + 4, maxlevel <= 29
-        FMM_HIP(hipMemcpyAsync(h, cnt, sizeof(uint32_t) * ncnt, hipMemcpyDeviceToHost, st));
-        FMM_HIP(hipMemcpyAsync(&tflag, c.d_tflag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        uint32_t h[TCNT_WORDS];  // ncnt <= 2 * maxlevel + 4 <= 62 counters, then the tree flag
+        FMM_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
         FMM_HIP(hipStreamSynchronize(st));
-        if (tflag) return false;  // the tree is not the reference's: build it exactly
+        if (h[TCNT_FLAG]) return false;  // the tree is not the reference's: build it exactly
         uint64_t maxf = 0;
         for (int it = 0; it <= iters; ++it) maxf = std::max<uint64_t>(maxf, h[2 + it]);
         bool over = h[0] > (uint64_t)c.cap_p2p || h[1] > (uint64_t)c.cap_m2l ||
@@ -231,12 +267,10 @@ bool build_lists_device(Ctx &c, double eta) {
         if (h[0] >= (uint64_t)c.cap_p2p) {
             c.cap_p2p = std::max<int64_t>(2 * (int64_t)h[0], 2 * c.cap_p2p);
             c.d_p2p.ensure(8 * (size_t)c.cap_p2p);
-            c.d_p2p_alt.ensure(8 * (size_t)c.cap_p2p);
         }
         if (h[1] >= (uint64_t)c.cap_m2l) {
             c.cap_m2l = std::max<int64_t>(2 * (int64_t)h[1], 2 * c.cap_m2l);
             c.d_m2l.ensure(8 * (size_t)c.cap_m2l);
-            c.d_m2l_alt.ensure(8 * (size_t)c.cap_m2l);
         }
         if (maxf >= (uint64_t)c.cap_front) {
             c.cap_front = std::max<int64_t>(2 * (int64_t)maxf, 2 * c.cap_front);
@@ -246,12 +280,31 @@ bool build_lists_device(Ctx &c, double eta) {
         if (c.cap_front > 0x7fffffffLL || c.cap_p2p > 0x7fffffffLL || c.cap_m2l > 0x7fffffffLL)
             fail(FMM4RB_ENOMEM, "interaction lists exceed 2^31 entries");
     }
-    // the sort scratch must also cover the longer list (the stream is idle here)
-    c.d_rs_scratch.ensure(4 * radix_scratch_words(std::max<int64_t>(std::max(c.np2p, c.nm2l), 1), 1));
-    sort_pairs_csr(c, c.d_p2p.as<uint64_t>(), c.d_p2p_alt.as<uint64_t>(), c.np2p,
-                   c.d_p2p_off.as<int32_t>(), &c.p2p_sorted);
-    sort_pairs_csr(c, c.d_m2l.as<uint64_t>(), c.d_m2l_alt.as<uint64_t>(), c.nm2l,
-                   c.d_m2l_off.as<int32_t>(), &c.m2l_sorted);
+    // Both lists sorted in one radix sort of two segments of L keys each (the shorter
+    // padded with keys above every pair), half the launches of two sorts; the
+    // buffers and scratch grow here only after a capacity retry (the stream is idle).
+    const int64_t L = std::max<int64_t>(std::max(c.np2p, c.nm2l), 1);
+    c.pair_stride = L;
+    c.d_pairs.ensure(2 * 8 * (size_t)L);
+    c.d_pairs_alt.ensure(2 * 8 * (size_t)L);
+    c.d_rs_scratch.ensure(4 * radix_scratch_words(L, 2));
+    uint64_t *pr = c.d_pairs.as<uint64_t>(), *pra = c.d_pairs_alt.as<uint64_t>();
+    hipLaunchKernelGGL(k_pack_pairs, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, st,
+                       c.d_p2p.as<uint64_t>(), c.np2p, c.d_m2l.as<uint64_t>(), c.nm2l, L,
+                       (1ull << (2 * c.sb)) - 1ull, pr);
+    FMM_LAUNCH_CHECK();
+    const bool alt = radix_sort(pr, pra, nullptr, nullptr, L, 2, 0, 2 * c.sb,
+                                c.d_rs_scratch.as<uint32_t>(), st);
+    uint64_t *r = alt ? pra : pr;
+    c.p2p_sorted = r;
+    c.m2l_sorted = r + L;
+    const int64_t nc = S.nc;
+    hipLaunchKernelGGL(k_csr, dim3((unsigned)((nc + 1 + 255) / 256)), dim3(256), 0, st, r, L, c.sb,
+                       nc, c.d_p2p_off.as<int32_t>());
+    FMM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_csr, dim3((unsigned)((nc + 1 + 255) / 256)), dim3(256), 0, st, r + L, L,
+                       c.sb, nc, c.d_m2l_off.as<int32_t>());
+    FMM_LAUNCH_CHECK();
     return true;
 }
 

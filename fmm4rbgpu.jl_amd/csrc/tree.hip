@@ -827,7 +827,7 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
     uint32_t *vals = c.d_vals.as<uint32_t>(), *vals_alt = c.d_vals_alt.as<uint32_t>();
     int32_t *seg = c.d_seg.as<int32_t>(), *seg_alt = c.d_seg_alt.as<int32_t>();
     uint32_t *rk = c.d_rk.as<uint32_t>(), *rk_alt = rk + 9 * (size_t)N;
-    uint32_t *flag = c.d_tflag.as<uint32_t>();
+    uint32_t *flag = c.d_tcnt.as<uint32_t>() + TCNT_FLAG;
     const uint32_t *perm;
     if (exact) {  // 64-bit keys of the exact coordinate order
         hipLaunchKernelGGL(k_make_keys, dim3(nb(N, 256)), dim3(256), 0, st, pos, N, keys, vals,
