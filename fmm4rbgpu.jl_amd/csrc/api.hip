@@ -426,7 +426,6 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
                                c.d_cnt.as<int32_t>(), tmp.as<unsigned long long>());
             FMM_LAUNCH_CHECK();
         }
-        FMM_HIP(hipMemcpyAsync(ls, tmp.p, 16, hipMemcpyDeviceToHost, sn));
     }
     FMM_HIP(hipStreamWaitEvent(st, c.evu[2], 0));  // join the upward pass
     FMM_HIP(hipEventRecord(c.ev[4], st));
@@ -443,6 +442,9 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
     FMM_HIP(hipEventRecord(c.ev[6], st));
     launch_leaf_l2p(c, amp, E, B);
     FMM_HIP(hipEventRecord(c.ev[7], st));
+    // the copy to pageable host memory blocks the host until the near-field stream
+    // reaches it, so it is issued only after every launch of the call
+    if (stats) FMM_HIP(hipMemcpyAsync(ls, c.d_bad.p, 16, hipMemcpyDeviceToHost, sn));
     FMM_HIP(hipStreamSynchronize(st));
     if (stats) FMM_HIP(hipStreamSynchronize(sn));
     c.tree_valid = true;
