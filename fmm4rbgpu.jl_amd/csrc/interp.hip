@@ -195,17 +195,32 @@ struct TransferGroup {
                                                           : 1;
 };
 
+// parent = transfer(child 1) + transfer(child 2), one parent per workgroup; when all
+// four components go through one transfer pass, both children's multipoles are
+// loaded into registers at the start (element e = tid + OP_T q), as in k_l2l.
 template <int NP>
 __global__ __launch_bounds__(OP_T) void k_m2m(const int32_t *__restrict__ parents,
                                               const double *__restrict__ ncoord,
                                               double *__restrict__ mp4) {
-    constexpr int NN = NP * NP * NP, TPT = (NN + OP_T - 1) / OP_T;
+    constexpr int NN = NP * NP * NP, TPT = (NN + OP_T - 1) / OP_T, NE = 4 * NN;
     constexpr int CG = TransferGroup<NP, 4>::CG;
+    constexpr bool PRE = CG == 4 && NE <= 8 * OP_T;
+    constexpr int EPT = PRE ? (NE + OP_T - 1) / OP_T : 1;
     __shared__ double Wm[3][NP][NP];  // [axis][child node][parent node]
     __shared__ double pn[3 * NP], cn[3 * NP];
     __shared__ double bin[CG][NN], ba[CG][NN], bb[CG][NN];
     const int tid = threadIdx.x;
     const int32_t P = parents[blockIdx.x];
+    double vc[2][EPT];
+    if constexpr (PRE) {
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                const int e = tid + OP_T * q;
+                vc[ch][q] = e < NE ? mp4[(int64_t)(2 * P + 1 + ch) * NE + e] : 0.0;
+            }
+    }
     if (tid < 3 * NP) pn[tid] = ncoord[(int64_t)P * 3 * NP + tid];
     double acc[TPT][4];
 #pragma unroll
@@ -223,9 +238,17 @@ __global__ __launch_bounds__(OP_T) void k_m2m(const int32_t *__restrict__ parent
         }
         for (int c0 = 0; c0 < 4; c0 += CG) {
             __syncthreads();
-            for (int e = tid; e < CG * NN; e += OP_T) {  // contiguous when CG = 4
-                const int n = e / CG, cc = e % CG;
-                bin[cc][n] = mp4[((int64_t)C * NN + n) * 4 + c0 + cc];
+            if constexpr (PRE) {
+#pragma unroll
+                for (int q = 0; q < EPT; ++q) {
+                    const int e = tid + OP_T * q;
+                    if (e < NE) bin[e % 4][e / 4] = vc[ch][q];
+                }
+            } else {
+                for (int e = tid; e < CG * NN; e += OP_T) {  // contiguous when CG = 4
+                    const int n = e / CG, cc = e % CG;
+                    bin[cc][n] = mp4[((int64_t)C * NN + n) * 4 + c0 + cc];
+                }
             }
             __syncthreads();
             for (int e = tid; e < CG * NN; e += OP_T) {  // contract x: ba[Pi, cj, ck]
@@ -272,18 +295,34 @@ __global__ __launch_bounds__(OP_T) void k_m2m(const int32_t *__restrict__ parent
 }
 
 // ------------------------------------------------------------------ L2L
+// child += transfer(parent), one child per workgroup.  The kernel waits on memory:
+// when all six components go through one transfer pass, both expansions are loaded
+// into registers at the start (element e = tid + OP_T q), before the node weights.
 template <int NP>
 __global__ __launch_bounds__(OP_T) void k_l2l(const int32_t *__restrict__ nodes,
                                               const double *__restrict__ ncoord,
                                               double *__restrict__ loc) {
-    constexpr int NN = NP * NP * NP;
+    constexpr int NN = NP * NP * NP, NE = 6 * NN;
     constexpr int CG = TransferGroup<NP, 6>::CG;
+    constexpr bool PRE = CG == 6 && NE <= 8 * OP_T;
+    constexpr int EPT = PRE ? (NE + OP_T - 1) / OP_T : 1;
     __shared__ double Wm[3][NP][NP];  // [axis][child node][parent node]
     __shared__ double pn[3 * NP], cn[3 * NP];
     __shared__ double bin[CG][NN], ba[CG][NN], bb[CG][NN];
     const int tid = threadIdx.x;
     const int32_t C = nodes[blockIdx.x];
     const int32_t P = (C - 1) / 2;
+    double *lc = loc + (int64_t)C * NE;
+    const double *lp = loc + (int64_t)P * NE;
+    double vl[EPT], vp[EPT];
+    if constexpr (PRE) {
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = tid + OP_T * q;
+            vl[q] = e < NE ? lc[e] : 0.0;
+            vp[q] = e < NE ? lp[e] : 0.0;
+        }
+    }
     if (tid < 3 * NP) {
         pn[tid] = ncoord[(int64_t)P * 3 * NP + tid];
         cn[tid] = ncoord[(int64_t)C * 3 * NP + tid];
@@ -294,10 +333,18 @@ __global__ __launch_bounds__(OP_T) void k_l2l(const int32_t *__restrict__ nodes,
         lgw<NP>(cn[tid], pn + d * NP, &Wm[d][ci][0]);
     }
     for (int c0 = 0; c0 < 6; c0 += CG) {
-        __syncthreads();
-        for (int e = tid; e < CG * NN; e += OP_T) {  // contiguous when CG = 6
-            const int n = e / CG, cc = e % CG;
-            bin[cc][n] = loc[((int64_t)P * NN + n) * 6 + c0 + cc];
+        if constexpr (PRE) {
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                const int e = tid + OP_T * q;
+                if (e < NE) bin[e % 6][e / 6] = vp[q];
+            }
+        } else {
+            __syncthreads();
+            for (int e = tid; e < CG * NN; e += OP_T) {  // contiguous when CG = 6
+                const int n = e / CG, cc = e % CG;
+                bin[cc][n] = lp[(int64_t)n * 6 + c0 + cc];
+            }
         }
         __syncthreads();
         for (int e = tid; e < CG * NN; e += OP_T) {  // ba[ci, Pj, Pk]
@@ -315,12 +362,26 @@ __global__ __launch_bounds__(OP_T) void k_l2l(const int32_t *__restrict__ nodes,
             bb[cc][n] = s;
         }
         __syncthreads();
-        for (int e = tid; e < CG * NN; e += OP_T) {  // child += contract z (contiguous)
-            const int n = e / CG, cc = e % CG;
-            const int ij = n % (NP * NP), ck = n / (NP * NP);
-            double s = 0.0;
-            for (int Pk = 0; Pk < NP; ++Pk) s += Wm[2][ck][Pk] * bb[cc][ij + NP * NP * Pk];
-            loc[((int64_t)C * NN + n) * 6 + c0 + cc] += s;
+        if constexpr (PRE) {  // child += contract z, element e = n * 6 + cc
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                const int e = tid + OP_T * q;
+                if (e < NE) {
+                    const int n = e / 6, cc = e % 6;
+                    const int ij = n % (NP * NP), ck = n / (NP * NP);
+                    double s = 0.0;
+                    for (int Pk = 0; Pk < NP; ++Pk) s += Wm[2][ck][Pk] * bb[cc][ij + NP * NP * Pk];
+                    lc[e] = vl[q] + s;
+                }
+            }
+        } else {
+            for (int e = tid; e < CG * NN; e += OP_T) {  // child += contract z (contiguous)
+                const int n = e / CG, cc = e % CG;
+                const int ij = n % (NP * NP), ck = n / (NP * NP);
+                double s = 0.0;
+                for (int Pk = 0; Pk < NP; ++Pk) s += Wm[2][ck][Pk] * bb[cc][ij + NP * NP * Pk];
+                lc[(int64_t)n * 6 + c0 + cc] += s;
+            }
         }
     }
 }
