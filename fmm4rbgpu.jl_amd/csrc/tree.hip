@@ -724,14 +724,31 @@ __global__ __launch_bounds__(TF_T) void k_tree_finish(
     for (int j = tid; j < c0; j += TF_T) rk[a0 + j] = Ls[0][j];
 }
 
-// Canonical within-leaf order: ascending particle id (rank sort in LDS).
+// Canonical within-leaf order: ascending particle id (rank sort in LDS); the same
+// workgroup then gathers the leaf's particles in that order (k_gather's copy).
+__device__ inline void gather_particle(int64_t i, int64_t p, const double *__restrict__ pos,
+                                       const double *__restrict__ mom, double *__restrict__ sp8,
+                                       float *__restrict__ sp4f) {
+    const double x0 = pos[3 * p], x1 = pos[3 * p + 1], x2 = pos[3 * p + 2];
+    const double m0 = mom[3 * p], m1 = mom[3 * p + 1], m2 = mom[3 * p + 2];
+    const double g = sqrt(1.0 + ((m0 * m0 + m1 * m1) + m2 * m2));  // interaction.jl:67
+    double4 *o = reinterpret_cast<double4 *>(sp8 + 8 * i);
+    o[0] = make_double4(x0, x1, x2, g);
+    o[1] = make_double4(m0, m1, m2, 0.0);
+    if (sp4f)
+        reinterpret_cast<float4 *>(sp4f)[i] = make_float4((float)g, (float)m0, (float)m1, (float)m2);
+}
+
 __global__ __launch_bounds__(256) void k_leaf_sort(const int32_t *__restrict__ leaves,
                                                    const int32_t *__restrict__ lo,
                                                    const int32_t *__restrict__ cnt,
                                                    const uint32_t *__restrict__ rkx,
                                                    const uint32_t *__restrict__ permx,
                                                    int32_t *__restrict__ parind,
-                                                   int32_t *__restrict__ leafof) {
+                                                   int32_t *__restrict__ leafof,
+                                                   const double *__restrict__ pos,
+                                                   const double *__restrict__ mom,
+                                                   double *__restrict__ sp8, float *__restrict__ sp4f) {
     __shared__ uint32_t s[MAX_N0];
     const int32_t k = leaves[blockIdx.x];
     const int32_t a = lo[k], c = cnt[k];
@@ -743,6 +760,7 @@ __global__ __launch_bounds__(256) void k_leaf_sort(const int32_t *__restrict__ l
         for (int m = 0; m < c; ++m) r += s[m] < v;
         parind[a + r] = (int32_t)v;
         leafof[a + j] = k;
+        gather_particle((int64_t)a + r, v, pos, mom, sp8, sp4f);
     }
 }
 
@@ -753,15 +771,7 @@ __global__ void k_gather(int64_t N, const int32_t *__restrict__ parind,
                          double *__restrict__ sp8, float *__restrict__ sp4f) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    int64_t p = parind[i];
-    double x0 = pos[3 * p], x1 = pos[3 * p + 1], x2 = pos[3 * p + 2];
-    double m0 = mom[3 * p], m1 = mom[3 * p + 1], m2 = mom[3 * p + 2];
-    double g = sqrt(1.0 + ((m0 * m0 + m1 * m1) + m2 * m2));  // interaction.jl:67
-    double4 *o = reinterpret_cast<double4 *>(sp8 + 8 * i);
-    o[0] = make_double4(x0, x1, x2, g);
-    o[1] = make_double4(m0, m1, m2, 0.0);
-    if (sp4f)
-        reinterpret_cast<float4 *>(sp4f)[i] = make_float4((float)g, (float)m0, (float)m1, (float)m2);
+    gather_particle(i, parind[i], pos, mom, sp8, sp4f);
 }
 
 // ------------------------------------------------------------------ refit
@@ -942,10 +952,8 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
     }
     const int nleaf = (int)S.leaves.size();
     hipLaunchKernelGGL(k_leaf_sort, dim3(nleaf), dim3(256), 0, st, c.d_leaves.as<int32_t>(), lo,
-                       cnt, rk, perm, c.d_parind.as<int32_t>(), c.d_leafof.as<int32_t>());
-    FMM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gather, dim3(nb(N, 256)), dim3(256), 0, st, N, c.d_parind.as<int32_t>(),
-                       pos, mom, c.d_sp8.as<double>(),
+                       cnt, rk, perm, c.d_parind.as<int32_t>(), c.d_leafof.as<int32_t>(), pos, mom,
+                       c.d_sp8.as<double>(),
                        c.precision == FMM4RB_MIXED ? c.d_sp8f.as<float>() : (float *)nullptr);
     FMM_LAUNCH_CHECK();
 }
