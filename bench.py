@@ -22,8 +22,8 @@ PEAK = {"mixed": 157.3e12, "fp64": 78.6e12}  # MI355X FP32 / FP64 vector peaks (
 FLOPS_PER_PAIR = 40  # one kernel evaluation, P2P particle pair or M2L node pair
 # HBM bytes per launch from two rocprofv3 PMC passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000,
 # calibrated on a 1 GiB copy), N=2^22, n=4: profiles/m2l_p2p_bunch_2e22_n4_valu.md
-TRAFFIC = {(1 << 22, 4, "mixed", "m2l"): 3.313e9, (1 << 22, 4, "mixed", "p2p"): 5.849e9,
-           (1 << 22, 4, "fp64", "m2l"): 1.530e10, (1 << 22, 4, "fp64", "p2p"): 4.059e9}
+TRAFFIC = {(1 << 22, 4, "mixed", "m2l"): 4.307e9, (1 << 22, 4, "mixed", "p2p"): 5.922e9,
+           (1 << 22, 4, "fp64", "m2l"): 1.539e10, (1 << 22, 4, "fp64", "p2p"): 4.044e9}
 
 
 def parse():
@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--large-N", type=int, default=25_600_000,
                     help="particles of the large uniform-ellipsoid case reported beside the "
                          "headline (FP64 and mixed forward, one FP64 reverse pass; 0 = skip)")
+    ap.add_argument("--adjoint-N", type=int, default=1 << 20,
+                    help="the reverse-mode configuration (bunch, seed 4) also timed on its own; "
+                         "0 skips it")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -215,6 +218,51 @@ def large_case(F, W, torch, N, n, N0, eta, lam):
     return out
 
 
+def adjoint_case(F, W, torch, N, n, N0, eta, lam, steps=3, warmup=1):
+    """The reverse-mode configuration: an N = 2^20 bunch (seed 4), L = sum w (|E|^2 +
+    |B|^2) with w ~ U(0, 1) (seed 40).  Per precision: the forward evaluation, the reverse
+    mode (tree and lists rebuilt) and the pullback after a forward call, each 1 warm-up
+    + `steps` timed calls, and each reverse mode's ratio to the forward of its own
+    precision.  Its own context, closed afterwards."""
+    dev = torch.device("cuda:0")
+    pos, mom = W.bunch(N, seed=4)
+    dpos, dmom = torch.from_numpy(pos).to(dev), torch.from_numpy(mom).to(dev)
+    E, B = torch.empty_like(dpos), torch.empty_like(dpos)
+    gpos, gmom = torch.empty_like(dpos), torch.empty_like(dpos)
+    w = torch.rand(N, 1, dtype=torch.float64, device=dev,
+                   generator=torch.Generator(device=dev).manual_seed(40))
+    out = {"workload": f"relativistic Gaussian bunch, N={N} (seed 4), loss sum w(|E|^2+|B|^2)",
+           "N": N, "n": n, "N0": N0, "eta": eta}
+
+    def clock(call):
+        for _ in range(warmup):
+            call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            call()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps * 1e3
+
+    with F.Context(0) as ctx:
+        fwd = lambda prec: ctx.field_device(N, dpos.data_ptr(), dmom.data_ptr(), E.data_ptr(),
+                                            B.data_ptr(), -1.0, lam, n, N0, eta, prec)
+        for prec in ("fp64", "mixed"):
+            ms_f = clock(lambda: fwd(prec))
+            Eb, Bb = 2 * w * E, 2 * w * B  # this precision's fields
+            args = (N, dpos.data_ptr(), dmom.data_ptr(), Eb.data_ptr(), Bb.data_ptr(),
+                    gpos.data_ptr(), gmom.data_ptr(), -1.0, lam, n, N0, eta, prec)
+            ms_v = clock(lambda: ctx.field_vjp_device(*args))
+            fwd(prec)
+            ms_p = clock(lambda: ctx.field_pullback_device(*args))
+            out[prec] = {"forward_ms": ms_f, "reverse_ms": ms_v, "pullback_ms": ms_p,
+                         "reverse_ratio_to_forward": ms_v / ms_f,
+                         "pullback_ratio_to_forward": ms_p / ms_f}
+    del dpos, dmom, E, B, gpos, gmom, w
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -369,6 +417,8 @@ def main():
         out["fp64"] = res["fp64"]
         out["speedup_vs_fp64"] = out["value"] / res["fp64"]["value"]
     ctx.close()
+    if not a.skip_fp64 and not a.skip_adjoint and a.adjoint_N > 0 and a.workload == "bunch":
+        out["adjoint_config"] = adjoint_case(F, W, torch, a.adjoint_N, a.n, a.N0, a.eta, lam)
     if a.large_N > 0:
         out["large_ellipsoid"] = large_case(F, W, torch, a.large_N, a.n, a.N0, a.eta, lam)
     if a.copy_probe_mib > 0:
