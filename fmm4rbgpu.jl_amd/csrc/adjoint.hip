@@ -1438,7 +1438,8 @@ __global__ void k_fill0(double *__restrict__ p, int64_t n) {
 // dl/d(bbox) per node: a_d (min) and b_d (max) through X_m = (c_m (b-a))/2 + (a+b)/2.
 __global__ void k_adj_bbox(int64_t nc, int np, const uint8_t *__restrict__ flags,
                            const double *__restrict__ Xbar, const double *__restrict__ Xbar2,
-                           const double *__restrict__ ctab, double *__restrict__ abar) {
+                           const double *__restrict__ Xbar3, const double *__restrict__ ctab,
+                           double *__restrict__ abar) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nc) return;
     for (int d = 0; d < 3; ++d) {
@@ -1446,7 +1447,7 @@ __global__ void k_adj_bbox(int64_t nc, int np, const uint8_t *__restrict__ flags
         if (flags[k] & 1)
             for (int m = 0; m < np; ++m) {
                 const int64_t j = (k * 3 + d) * np + m;
-                const double xb = Xbar[j] + Xbar2[j];
+                const double xb = (Xbar[j] + Xbar2[j]) + Xbar3[j];
                 sa += xb * ((1.0 - ctab[m]) / 2.0);
                 sbm += xb * ((1.0 + ctab[m]) / 2.0);
             }
@@ -1456,41 +1457,67 @@ __global__ void k_adj_bbox(int64_t nc, int np, const uint8_t *__restrict__ flags
 }
 
 // Each particle collects the bbox gradients of the ancestors where it is the
-// argmin / argmax, then the mean-momentum gradient; results go to the caller's
-// original particle order.
-__global__ void k_adj_finish(int64_t N, const int32_t *__restrict__ parind,
-                             const int32_t *__restrict__ leafof, const int32_t *__restrict__ ext,
-                             const double *__restrict__ abar, const double *__restrict__ pbar,
-                             const double *__restrict__ gx, const double *__restrict__ gp,
-                             const double *__restrict__ gx2, const double *__restrict__ gp2,
-                             double *__restrict__ gpos, double *__restrict__ gmom) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const int32_t p = parind[i];
-    double g[3] = {gx[3 * i] + gx2[3 * i], gx[3 * i + 1] + gx2[3 * i + 1],
-                   gx[3 * i + 2] + gx2[3 * i + 2]};
-    int32_t k = leafof[i];
-    for (;;) {
-        for (int qd = 0; qd < 6; ++qd)
-            if (ext[6 * (int64_t)k + qd] == p) g[qd % 3] += abar[6 * (int64_t)k + qd];
-        if (k == 0) break;
-        k = (k - 1) / 2;
+// argmin / argmax (leaf to root), then the mean-momentum gradient; results go to the
+// caller's original particle order.  One workgroup per leaf: the ancestors' extreme
+// particles and their gradients (6 per level, the same for every particle of the
+// leaf) are staged in LDS once.
+constexpr int FIN_T = 64, FIN_MAXL = 32;  // leaf depth <= 29
+__global__ __launch_bounds__(FIN_T) void k_adj_finish(
+    const int32_t *__restrict__ leaves, const int32_t *__restrict__ lo,
+    const int32_t *__restrict__ cnt, const int32_t *__restrict__ parind,
+    const int32_t *__restrict__ ext, const double *__restrict__ abar,
+    const double *__restrict__ pbar, int64_t N, const double *__restrict__ gx,
+    const double *__restrict__ gp, const double *__restrict__ gx2, const double *__restrict__ gp2,
+    double *__restrict__ gpos, double *__restrict__ gmom) {
+    __shared__ int32_t sext[FIN_MAXL * 6];
+    __shared__ double sab[FIN_MAXL * 6];
+    const int32_t k = leaves[blockIdx.x];
+    const int64_t a = lo[k];
+    const int c = cnt[k];
+    const int depth = 31 - __clz(k + 1);  // level of the leaf: ancestors 0 .. depth
+    const int na = 6 * (depth + 1);
+    for (int j = threadIdx.x; j < na; j += FIN_T) {  // entry j: ancestor j / 6 levels up
+        int32_t anc = k;
+        for (int u = 0; u < j / 6; ++u) anc = (anc - 1) / 2;
+        sext[j] = ext[6 * (int64_t)anc + j % 6];
+        sab[j] = abar[6 * (int64_t)anc + j % 6];
     }
-    for (int d = 0; d < 3; ++d) {
-        gpos[3 * (int64_t)p + d] = g[d];
-        gmom[3 * (int64_t)p + d] = (gp[3 * i + d] + gp2[3 * i + d]) + pbar[d] / (double)N;
+    __syncthreads();
+    const double p0 = pbar[0] / (double)N, p1 = pbar[1] / (double)N, p2 = pbar[2] / (double)N;
+    for (int t = threadIdx.x; t < c; t += FIN_T) {
+        const int64_t i = a + t;
+        const int32_t p = parind[i];
+        double g[3] = {gx[3 * i] + gx2[3 * i], gx[3 * i + 1] + gx2[3 * i + 1],
+                       gx[3 * i + 2] + gx2[3 * i + 2]};
+        for (int j = 0; j < na; ++j)
+            if (sext[j] == p) g[j % 3] += sab[j];
+        gpos[3 * (int64_t)p] = g[0];
+        gpos[3 * (int64_t)p + 1] = g[1];
+        gpos[3 * (int64_t)p + 2] = g[2];
+        gmom[3 * (int64_t)p] = (gp[3 * i] + gp2[3 * i]) + p0;
+        gmom[3 * (int64_t)p + 1] = (gp[3 * i + 1] + gp2[3 * i + 1]) + p1;
+        gmom[3 * (int64_t)p + 2] = (gp[3 * i + 2] + gp2[3 * i + 2]) + p2;
     }
 }
 
-// Fixed-order sum of the per-target mean-momentum partials.
-__global__ __launch_bounds__(256) void k_adj_pbar(const double *__restrict__ part, int64_t nc,
-                                                  double *__restrict__ out) {
-    __shared__ double s[3][256];
-    double a[3] = {0.0, 0.0, 0.0};
-    for (int64_t k = threadIdx.x; k < nc; k += 256)
-        for (int d = 0; d < 3; ++d) a[d] += part[3 * k + d];
-    for (int d = 0; d < 3; ++d) s[d][threadIdx.x] = a[d];
-    for (int o = 128; o > 0; o >>= 1) {
+// Fixed-order sum of the per-target mean-momentum partials: 1024 lanes each sum a
+// contiguous stretch with four running sums, then a tree over the lanes.
+constexpr int PB_T = 1024;
+__global__ __launch_bounds__(PB_T) void k_adj_pbar(const double *__restrict__ part, int64_t nc,
+                                                   double *__restrict__ out) {
+    __shared__ double s[3][PB_T];
+    const int64_t per = (nc + PB_T - 1) / PB_T;
+    const int64_t b = threadIdx.x * per, e = min(nc, b + per);
+    double a[4][3] = {};
+    int64_t k = b;
+    for (; k + 4 <= e; k += 4)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            for (int d = 0; d < 3; ++d) a[u][d] += part[3 * (k + u) + d];
+    for (; k < e; ++k)
+        for (int d = 0; d < 3; ++d) a[0][d] += part[3 * k + d];
+    for (int d = 0; d < 3; ++d) s[d][threadIdx.x] = (a[0][d] + a[1][d]) + (a[2][d] + a[3][d]);
+    for (int o = PB_T / 2; o > 0; o >>= 1) {
         __syncthreads();
         if (threadIdx.x < (unsigned)o)
             for (int d = 0; d < 3; ++d) s[d][threadIdx.x] += s[d][threadIdx.x + o];
@@ -1533,6 +1560,7 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     c.d_mbar.ensure(4 * (size_t)NN * nc * 8);
     c.d_xbar.ensure(3 * (size_t)np * nc * 8);
     c.d_xbar2.ensure(3 * (size_t)np * nc * 8);
+    c.d_xbar3.ensure(3 * (size_t)np * nc * 8);
     c.d_pbar.ensure((3 * (size_t)nc + 3) * 8);
     c.d_abar.ensure(6 * (size_t)nc * 8);
     c.d_costab.ensure(MAX_NP * 8);
@@ -1540,11 +1568,10 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
         c.costab_host[i] = i < c.np ? cos(M_PI * (double)i / (double)c.n) : 0.0;  // cheb2.jl:4
     FMM_HIP(hipMemcpyAsync(c.d_costab.p, c.costab_host, sizeof c.costab_host, hipMemcpyHostToDevice,
                            st));
-    transpose_lists(c, c.p2p_sorted, c.np2p, c.d_p2pT, c.d_p2pT_alt, c.d_p2pT_off, &c.p2pT_sorted);
-    // the target side's per-pair source-node sums (n <= 6) go to the transposed position
+    // both lists regrouped by source in one sort; the target side's per-pair
+    // source-node sums (n <= 6) go to the transposed position
     const bool ysplit = c.np <= 7;
-    transpose_lists(c, c.m2l_sorted, c.nm2l, c.d_m2lT, c.d_m2lT_alt, c.d_m2lT_off, &c.m2lT_sorted,
-                    ysplit ? &c.d_m2lT_pos : nullptr);
+    transpose_lists_joint(c, ysplit ? &c.d_m2lT_pos : nullptr);
     if (ysplit) c.d_ysp.ensure(3 * (size_t)c.np * (size_t)std::max<int64_t>(c.nm2l, 1) * 8);
     const uint32_t *posT = ysplit ? c.d_m2lT_pos.as<uint32_t>() : nullptr;
     double *ysp = ysplit ? c.d_ysp.as<double>() : nullptr;
@@ -1606,8 +1633,10 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     const int64_t nx = 3 * np * nc;
     hipLaunchKernelGGL(k_fill0, dim3(nbk(nx, 256)), dim3(256), 0, st, xbar, nx);
     FMM_LAUNCH_CHECK();
-    double *xbar2 = c.d_xbar2.as<double>();
+    double *xbar2 = c.d_xbar2.as<double>(), *xbar3 = c.d_xbar3.as<double>();
     hipLaunchKernelGGL(k_fill0, dim3(nbk(nx, 256)), dim3(256), 0, st, xbar2, nx);
+    FMM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fill0, dim3(nbk(nx, 256)), dim3(256), 0, st, xbar3, nx);
     FMM_LAUNCH_CHECK();
     double *loc = c.d_loc.as<double>(), *pbar = c.d_pbar.as<double>();
     // mixed precision: the M2L^T kernels read the float4 copy of the multipoles,
@@ -1631,6 +1660,38 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
         ADJ_NP_SWITCH(c.np, ADJ_L2LV);
         FMM_LAUNCH_CHECK();
     }
+    // 3. geometry of L2P^T and L2L^T (needs lbar and the final forward locals;
+    //    latency-bound launches) on the third stream, into xbar3: with saved_locals
+    //    the forward locals are in loc already, so it runs from here, beside P2P^T and
+    //    the M2L^T target side; otherwise after the forward L2L (below), beside the
+    //    M2L^T source side.  The L2L^T geometry of all levels in two launches (parents
+    //    on even, then odd levels).
+    hipStream_t sg = c.third_stream();
+    const int64_t npar = S.split_off[S.ml];  // split nodes of levels 0 .. ml-1
+    const int32_t *allpar = c.d_lvl_split.as<int32_t>();
+    auto launch_geometry = [&]() {
+        FMM_HIP(hipEventRecord(c.evy[0], st));
+        FMM_HIP(hipStreamWaitEvent(sg, c.evy[0], 0));
+#define ADJ_L2PG_T(NP, LT)                                                                         \
+    hipLaunchKernelGGL((k_adj_leaf<NP, 0, 1, LT>), dim3(nleaf), dim3(LT), 0, sg, leaves, lo, cnt,  \
+                       sp8, ncoord, loc, ob, lbar, gx, gp, xbar3)
+#define ADJ_L2PG(NP)                    \
+    if (S.maxleaf <= 64) {              \
+        ADJ_L2PG_T(NP, 64);             \
+    } else {                            \
+        ADJ_L2PG_T(NP, AT);             \
+    }
+        ADJ_NP_SWITCH(c.np, ADJ_L2PG);
+        FMM_LAUNCH_CHECK();
+        for (int parity = 0; parity < 2 && npar > 0; ++parity) {
+#define ADJ_L2LG(NP)                                                                               \
+    hipLaunchKernelGGL((k_adj_transfer<NP, 0, 1>), dim3((unsigned)npar), dim3(AT), 0, sg, allpar, \
+                       ncoord, lbar, loc, lbar, xbar3, parity)
+            ADJ_NP_SWITCH(c.np, ADJ_L2LG);
+            FMM_LAUNCH_CHECK();
+        }
+    };
+    if (saved_locals) launch_geometry();
     // 2. forward M2L fused with the M2L^T target side, then the forward L2L; with
     //    saved_locals the final forward locals of the preceding field call are in
     //    loc already and only the M2L^T target side runs
@@ -1658,37 +1719,12 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
         ADJ_NP_SWITCH(c.np, ADJ_M2LT);
         FMM_LAUNCH_CHECK();
         launch_l2l(c);
+        launch_geometry();
     }
     FMM_HIP(hipEventRecord(c.ev[8], st));
-    // 3. geometry of L2P^T and L2L^T (needs the final forward locals; latency-bound
-    //    launches): on the third stream, beside the M2L^T source side (4.), which
-    //    adds its node-coordinate terms into xbar2; the L2L^T geometry of all
-    //    levels in two launches (parents on even, then odd levels)
-    hipStream_t sg = c.third_stream();
-    FMM_HIP(hipEventRecord(c.evy[0], st));
-    FMM_HIP(hipStreamWaitEvent(sg, c.evy[0], 0));
-#define ADJ_L2PG_T(NP, LT)                                                                         \
-    hipLaunchKernelGGL((k_adj_leaf<NP, 0, 1, LT>), dim3(nleaf), dim3(LT), 0, sg, leaves, lo, cnt,  \
-                       sp8, ncoord, loc, ob, lbar, gx, gp, xbar)
-#define ADJ_L2PG(NP)                    \
-    if (S.maxleaf <= 64) {              \
-        ADJ_L2PG_T(NP, 64);             \
-    } else {                            \
-        ADJ_L2PG_T(NP, AT);             \
-    }
-    ADJ_NP_SWITCH(c.np, ADJ_L2PG);
-    FMM_LAUNCH_CHECK();
-    const int64_t npar = S.split_off[S.ml];  // split nodes of levels 0 .. ml-1
-    const int32_t *allpar = c.d_lvl_split.as<int32_t>();
-    for (int parity = 0; parity < 2 && npar > 0; ++parity) {
-#define ADJ_L2LG(NP)                                                                               \
-    hipLaunchKernelGGL((k_adj_transfer<NP, 0, 1>), dim3((unsigned)npar), dim3(AT), 0, sg, allpar, \
-                       ncoord, lbar, loc, lbar, xbar, parity)
-        ADJ_NP_SWITCH(c.np, ADJ_L2LG);
-        FMM_LAUNCH_CHECK();
-    }
+    FMM_HIP(hipStreamWaitEvent(sg, c.ev[8], 0));  // pbar: the target side's partials
     // the mean-momentum partials of the M2L^T target side, summed (one workgroup)
-    hipLaunchKernelGGL(k_adj_pbar, dim3(1), dim3(256), 0, sg, pbar, nc, pbar + 3 * nc);
+    hipLaunchKernelGGL(k_adj_pbar, dim3(1), dim3(PB_T), 0, sg, pbar, nc, pbar + 3 * nc);
     FMM_LAUNCH_CHECK();
     FMM_HIP(hipEventRecord(c.evy[1], sg));
     // 4. M2L^T source side
@@ -1747,12 +1783,12 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
     // bounding boxes -> extreme particles, mean momentum, output order
     FMM_HIP(hipStreamWaitEvent(st, c.evy[1], 0));  // join the third stream (xbar, gx, gp)
     hipLaunchKernelGGL(k_adj_bbox, dim3(nbk(nc, 128)), dim3(128), 0, st, nc, c.np,
-                       c.d_flags.as<uint8_t>(), xbar, xbar2, c.d_costab.as<double>(),
+                       c.d_flags.as<uint8_t>(), xbar, xbar2, xbar3, c.d_costab.as<double>(),
                        c.d_abar.as<double>());
     FMM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_adj_finish, dim3(nbk(N, 256)), dim3(256), 0, st, N,
-                       c.d_parind.as<int32_t>(), c.d_leafof.as<int32_t>(), c.d_ext.as<int32_t>(),
-                       c.d_abar.as<double>(), pbar + 3 * nc, gx, gp, gx2, gp2, gpos, gmom);
+    hipLaunchKernelGGL(k_adj_finish, dim3(nleaf), dim3(FIN_T), 0, st, leaves, lo, cnt,
+                       c.d_parind.as<int32_t>(), c.d_ext.as<int32_t>(), c.d_abar.as<double>(),
+                       pbar + 3 * nc, N, gx, gp, gx2, gp2, gpos, gmom);
     FMM_LAUNCH_CHECK();
 }
 

@@ -106,8 +106,8 @@ static int ctx_bufs(Ctx &c, DevBuf **out) {
             &c.d_vals, &c.d_vals_alt, &c.d_rs_scratch, &c.d_seg, &c.d_seg_alt,
             &c.d_rk, &c.d_pivot, &c.d_bsum, &c.d_axis, &c.d_parind, &c.d_leafof,
             &c.d_bbox, &c.d_geo, &c.d_ncoord, &c.d_costab, &c.d_ext, &c.d_sp8, &c.d_sp8f,
-            &c.d_ob, &c.d_gx, &c.d_gp, &c.d_gx2, &c.d_gp2, &c.d_xbar2, &c.d_lbar, &c.d_mbar,
-            &c.d_xbar, &c.d_pbar, &c.d_abar, &c.d_p2pT, &c.d_p2pT_alt, &c.d_m2lT, &c.d_m2lT_alt,
+            &c.d_ob, &c.d_gx, &c.d_gp, &c.d_gx2, &c.d_gp2, &c.d_xbar2, &c.d_xbar3, &c.d_lbar, &c.d_mbar,
+            &c.d_xbar, &c.d_pbar, &c.d_abar, &c.d_p2pT, &c.d_p2pT_alt,
             &c.d_p2pT_off, &c.d_m2lT_off, &c.d_front0, &c.d_front1, &c.d_tcnt, &c.d_pairs, &c.d_pairs_alt, &c.d_p2p,
             &c.d_p2p_alt, &c.d_m2l, &c.d_m2l_alt, &c.d_p2p_off, &c.d_m2l_off, &c.d_mp4,
             &c.d_mp4f, &c.d_loc, &c.d_near, &c.d_m2lT_pos, &c.d_ysp};

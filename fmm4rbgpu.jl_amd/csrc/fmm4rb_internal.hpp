@@ -170,6 +170,10 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
                  double *gmom, bool saved_locals);
 void transpose_lists(Ctx &c, const uint64_t *sorted, int64_t n, DevBuf &kT, DevBuf &kT_alt,
                      DevBuf &offT, uint64_t **sortedT, DevBuf *pos = nullptr);
+// Both of the context's lists regrouped by source in one two-segment sort:
+// p2pT_sorted / m2lT_sorted and their CSR offsets per source; with pos, pos[i] = the
+// transposed position of forward M2L pair i.
+void transpose_lists_joint(Ctx &c, DevBuf *pos);
 void launch_bruteforce_vjp(const double *pos, const double *mom, int64_t N, const double *ob,
                            double *gpos, double *gmom, hipStream_t st);
 
@@ -244,7 +248,8 @@ struct Ctx {
     DevBuf d_ob, d_gx, d_gp, d_lbar, d_mbar, d_xbar, d_pbar, d_abar;
     DevBuf d_gx2, d_gp2;  // P2P^T gradients (written on aux)
     DevBuf d_xbar2;       // node-coordinate gradients of the M2L^T source side
-    DevBuf d_p2pT, d_p2pT_alt, d_m2lT, d_m2lT_alt, d_p2pT_off, d_m2lT_off;
+    DevBuf d_xbar3;       // node-coordinate gradients of the L2P^T / L2L^T geometry
+    DevBuf d_p2pT, d_p2pT_alt, d_p2pT_off, d_m2lT_off;  // both transposed lists, side by side
     DevBuf d_m2lT_pos, d_ysp;  // transposed position of each M2L pair; per-pair source sums
     uint64_t *p2pT_sorted = nullptr, *m2lT_sorted = nullptr;
     // debug snapshots

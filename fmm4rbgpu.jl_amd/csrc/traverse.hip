@@ -182,16 +182,6 @@ void sort_pairs_csr(Ctx &c, uint64_t *keys, uint64_t *keys_alt, int64_t n, int32
     FMM_LAUNCH_CHECK();
 }
 
-// (t << sb) | s -> (s << sb) | t; with idx, idx[i] = i (the forward position)
-__global__ void k_transpose_keys(const uint64_t *__restrict__ in, int64_t n, int sb,
-                                 uint64_t *__restrict__ out, uint32_t *__restrict__ idx) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t k = in[i], mask = (1ull << sb) - 1ull;
-    out[i] = ((k & mask) << sb) | (k >> sb);
-    if (idx) idx[i] = (uint32_t)i;
-}
-
 // pos[fwd[e]] = e: the transposed position of every forward pair
 __global__ void k_scatter_pos(const uint32_t *__restrict__ fwd, int64_t n,
                               uint32_t *__restrict__ pos) {
@@ -199,31 +189,56 @@ __global__ void k_scatter_pos(const uint32_t *__restrict__ fwd, int64_t n,
     if (e < n) pos[fwd[e]] = (uint32_t)e;
 }
 
-// Pairs regrouped by source: keys (s << sb) | t sorted, CSR offsets per source.
-// With pos: pos[i] = position in the transposed list of forward pair i.
-void transpose_lists(Ctx &c, const uint64_t *sorted, int64_t n, DevBuf &kT, DevBuf &kT_alt,
-                     DevBuf &offT, uint64_t **sortedT, DevBuf *pos) {
-    kT.ensure(8 * (size_t)std::max<int64_t>(n, 1));
-    kT_alt.ensure(8 * (size_t)std::max<int64_t>(n, 1));
-    offT.ensure(4 * (size_t)(c.shape.nc + 1));
-    uint32_t *v = nullptr, *va = nullptr, *vs = nullptr;
-    if (pos) {
-        c.d_vals.ensure(4 * (size_t)std::max<int64_t>(n, 1));
-        c.d_vals_alt.ensure(4 * (size_t)std::max<int64_t>(n, 1));
-        pos->ensure(4 * (size_t)std::max<int64_t>(n, 1));
-        v = c.d_vals.as<uint32_t>();
-        va = c.d_vals_alt.as<uint32_t>();
-    }
-    if (n > 0) {
-        hipLaunchKernelGGL(k_transpose_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                           c.stream, sorted, n, c.sb, kT.as<uint64_t>(), v);
-        FMM_LAUNCH_CHECK();
-    }
-    sort_pairs_csr(c, kT.as<uint64_t>(), kT_alt.as<uint64_t>(), n, offT.as<int32_t>(), sortedT, v,
-                   va, pos ? &vs : nullptr);
-    if (pos && n > 0) {
-        hipLaunchKernelGGL(k_scatter_pos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                           c.stream, vs, n, pos->as<uint32_t>());
+// Both lists transposed ((t << sb) | s -> (s << sb) | t) side by side, each padded to
+// L keys with pad, and each pair's position within its list as the value.
+__global__ void k_transpose_pack(const uint64_t *__restrict__ p2p, int64_t np2p,
+                                 const uint64_t *__restrict__ m2l, int64_t nm2l, int64_t L,
+                                 int sb, uint64_t pad, uint64_t *__restrict__ out,
+                                 uint32_t *__restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L) return;
+    const uint64_t mask = (1ull << sb) - 1ull;
+    const uint64_t a = i < np2p ? p2p[i] : 0ull, b = i < nm2l ? m2l[i] : 0ull;
+    out[i] = i < np2p ? ((a & mask) << sb) | (a >> sb) : pad;
+    out[L + i] = i < nm2l ? ((b & mask) << sb) | (b >> sb) : pad;
+    idx[i] = (uint32_t)i;
+    idx[L + i] = (uint32_t)i;
+}
+
+// Pairs regrouped by source, both lists in one two-segment sort (as the forward
+// lists): keys (s << sb) | t, CSR offsets per source; with pos, pos[i] = the position
+// in the transposed M2L list of forward M2L pair i.
+void transpose_lists_joint(Ctx &c, DevBuf *pos) {
+    hipStream_t st = c.stream;
+    const int64_t L = std::max<int64_t>(std::max(c.np2p, c.nm2l), 1), nc = c.shape.nc;
+    c.d_p2pT.ensure(2 * 8 * (size_t)L);
+    c.d_p2pT_alt.ensure(2 * 8 * (size_t)L);
+    c.d_vals.ensure(2 * 4 * (size_t)L);
+    c.d_vals_alt.ensure(2 * 4 * (size_t)L);
+    c.d_p2pT_off.ensure(4 * (size_t)(nc + 1));
+    c.d_m2lT_off.ensure(4 * (size_t)(nc + 1));
+    c.d_rs_scratch.ensure(4 * radix_scratch_words(L, 2));
+    uint64_t *k = c.d_p2pT.as<uint64_t>(), *ka = c.d_p2pT_alt.as<uint64_t>();
+    uint32_t *v = c.d_vals.as<uint32_t>(), *va = c.d_vals_alt.as<uint32_t>();
+    hipLaunchKernelGGL(k_transpose_pack, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, st,
+                       c.p2p_sorted, c.np2p, c.m2l_sorted, c.nm2l, L, c.sb,
+                       (1ull << (2 * c.sb)) - 1ull, k, v);
+    FMM_LAUNCH_CHECK();
+    const bool alt = radix_sort(k, ka, v, va, L, 2, 0, 2 * c.sb, c.d_rs_scratch.as<uint32_t>(), st);
+    uint64_t *r = alt ? ka : k;
+    const uint32_t *vs = alt ? va : v;
+    c.p2pT_sorted = r;
+    c.m2lT_sorted = r + L;
+    hipLaunchKernelGGL(k_csr, dim3((unsigned)((nc + 1 + 255) / 256)), dim3(256), 0, st, r, L, c.sb,
+                       nc, c.d_p2pT_off.as<int32_t>());
+    FMM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_csr, dim3((unsigned)((nc + 1 + 255) / 256)), dim3(256), 0, st, r + L, L,
+                       c.sb, nc, c.d_m2lT_off.as<int32_t>());
+    FMM_LAUNCH_CHECK();
+    if (pos && c.nm2l > 0) {
+        pos->ensure(4 * (size_t)c.nm2l);
+        hipLaunchKernelGGL(k_scatter_pos, dim3((unsigned)((c.nm2l + 255) / 256)), dim3(256), 0, st,
+                           vs + L, c.nm2l, pos->as<uint32_t>());
         FMM_LAUNCH_CHECK();
     }
 }
