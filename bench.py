@@ -221,7 +221,8 @@ def large_case(F, W, torch, N, n, N0, eta, lam):
 def adjoint_case(F, W, torch, N, n, N0, eta, lam, steps=3, warmup=1):
     """The reverse-mode configuration: an N = 2^20 bunch (seed 4), L = sum w (|E|^2 +
     |B|^2) with w ~ U(0, 1) (seed 40).  Per precision: the forward evaluation, the reverse
-    mode (tree and lists rebuilt) and the pullback after a forward call, each 1 warm-up
+    mode (tree and lists rebuilt) and the pullback after a forward call (which keeps that
+    call's tree, lists and expansions), each 1 warm-up
     + `steps` timed calls, and each reverse mode's ratio to the forward of its own
     precision.  Its own context, closed afterwards."""
     dev = torch.device("cuda:0")
@@ -345,7 +346,8 @@ def main():
             "gpos": W.rel_l2(grads["mixed"][0], grads["fp64"][0]),
             "gmom": W.rel_l2(grads["mixed"][1], grads["fp64"][1])}
         # the rrule's pullback: after a forward call on the same inputs, the reverse
-        # mode reuses its final local expansions (tree and lists still recomputed)
+        # mode reuses that call's tree, lists, multipoles and final local expansions
+        # (repeated pullbacks of one forward call, as for several cotangents)
         for prec in ("fp64", "mixed"):
             gpos, gmom = torch.empty_like(dpos), torch.empty_like(dpos)
             Ef, Bf = torch.empty_like(dpos), torch.empty_like(dpos)

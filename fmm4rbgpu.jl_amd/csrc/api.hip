@@ -489,7 +489,10 @@ static void run_field(Ctx &c, int64_t N, const double *pos, const double *mom, d
 // Reverse mode: tree, lists and upward pass rebuilt, then the adjoint.  The forward
 // M2L and L2L are recomputed inside it, except for the pullback (saved_locals),
 // which takes the final local expansions of the preceding field call on the same
-// inputs from loc (verified by parameters and the input digest).
+// inputs from loc (verified by parameters and the input digest).  The pullback also
+// keeps that call's tree, lists and multipoles when the context still holds them
+// (tree_valid with the same N, N0 and eta): the build is deterministic, so
+// rebuilding would reproduce them bit for bit, and the adjoint only reads them.
 static void run_vjp(Ctx &c, int64_t N, const double *pos, const double *mom, double charge,
                     double lambda, int n, int64_t N0, double eta, int precision,
                     const double *Ebar, const double *Bbar, double *gpos, double *gmom,
@@ -515,11 +518,20 @@ static void run_vjp(Ctx &c, int64_t N, const double *pos, const double *mom, dou
     // a refused pullback leaves the context as it was (same parameters: prepare
     // changed nothing); past this point the tree is being rebuilt
     if (saved_locals && c.in_digest != c.loc_digest) fail(FMM4RB_EINVAL, nofwd);
-    c.tree_valid = false;
+    const bool keep_tree = saved_locals && c.tree_valid && c.tree_N == N && c.tree_N0 == N0 &&
+                           c.tree_eta == eta;
     FMM_HIP(hipEventRecord(c.ev[1], st));
-    tree_and_lists(c, pos, mom, N, N0, eta, c.ev[2]);
-    FMM_HIP(hipEventRecord(c.ev[3], st));
-    FMM_HIP(hipStreamWaitEvent(st, c.evu[2], 0));  // join the upward pass
+    if (keep_tree) {  // tree, lists and multipoles of the forward call (zero stage times)
+        FMM_HIP(hipEventRecord(c.ev[2], st));
+        FMM_HIP(hipEventRecord(c.ev[3], st));
+        FMM_HIP(hipEventRecord(c.evu[1], st));
+        FMM_HIP(hipEventRecord(c.evu[2], st));
+    } else {
+        c.tree_valid = false;
+        tree_and_lists(c, pos, mom, N, N0, eta, c.ev[2]);
+        FMM_HIP(hipEventRecord(c.ev[3], st));
+        FMM_HIP(hipStreamWaitEvent(st, c.evu[2], 0));  // join the upward pass
+    }
     FMM_HIP(hipEventRecord(c.ev[4], st));
     FMM_HIP(hipEventRecord(c.ev[5], st));
     run_adjoint(c, Ebar, Bbar, amp, gpos, gmom, saved_locals);

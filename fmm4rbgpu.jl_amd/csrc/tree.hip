@@ -201,7 +201,10 @@ __global__ void k_make_keys(const double *__restrict__ pos, int64_t N, uint64_t 
 // non-decreasing in x, so a stable sort of (key, id) from id order gives the (x, id)
 // order everywhere except inside runs of equal keys that hold different coordinates
 // (positions closer than the range / 2^32); k_presort_mark / k_presort_fix order
-// those runs.  4 radix passes over 4-byte keys instead of 8 over 8-byte ones.
+// those runs.  4 radix passes over 4-byte keys instead of 8 over 8-byte ones.  (24-bit
+// keys, 3 passes, were slower at N = 2^22: a Gaussian's central keys then hold ~1
+// particle each, and the run fix-up grew from 45 to 307 us, more than the pass saved.)
+constexpr int PRESORT_BITS = 32;
 constexpr double PRESORT_QMAX = 4294967040.0;  // 2^32 - 256: the product's rounding stays below 2^32
 constexpr int PRESORT_RUN_CAP = 256;           // longest run the fix-up orders
 
@@ -349,32 +352,35 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_count(
     __shared__ uint32_t s[PT_WAVES];
     const int d = blockIdx.y;
     const int64_t base = (int64_t)blockIdx.x * PT_TILE;
-    // every entry's segment and rank triple first (independent loads), then the
-    // cluster's split data
-    int32_t kk[PT_ITEMS];
-    uint32_t v[PT_ITEMS][3];
+    // every entry's segment first, then its cluster's split data, then only the one
+    // rank the side needs (r_a of the cluster's axis): 8 bytes per entry instead of 16
+    int32_t kk[PT_ITEMS], ax[PT_ITEMS];
+    uint32_t pv[PT_ITEMS], ra[PT_ITEMS];
     const uint32_t *r0 = rk + (int64_t)(3 * d) * N;
 #pragma unroll
     for (int j = 0; j < PT_ITEMS; ++j) {
         const int64_t i = base + j * PT_THREADS + threadIdx.x;
-        kk[j] = -1;
-        v[j][0] = v[j][1] = v[j][2] = 0u;
-        if (i < N) {
-            kk[j] = seg[i];
+        kk[j] = i < N ? seg[i] : -1;
+    }
 #pragma unroll
-            for (int e = 0; e < 3; ++e) v[j][e] = r0[e * N + i];
+    for (int j = 0; j < PT_ITEMS; ++j) {
+        const int32_t k = kk[j];
+        ax[j] = -1;
+        pv[j] = 0u;
+        if (k >= 0 && splitting(k, l, flags)) {
+            ax[j] = axis[k];
+            pv[j] = pivot[k];
         }
+    }
+#pragma unroll
+    for (int j = 0; j < PT_ITEMS; ++j) {
+        const int64_t i = base + j * PT_THREADS + threadIdx.x;
+        ra[j] = ax[j] >= 0 ? r0[ax[j] * N + i] : 0u;
     }
     uint32_t c = 0;
 #pragma unroll
     for (int j = 0; j < PT_ITEMS; ++j) {
-        const int32_t k = kk[j];
-        uint32_t f = 0;
-        if (k >= 0 && splitting(k, l, flags)) {  // r_a below the cluster's pivot rank
-            const int a = axis[k];
-            const uint32_t ra = a == 0 ? v[j][0] : (a == 1 ? v[j][1] : v[j][2]);
-            f = ra < pivot[k] ? 1u : 0u;
-        }
+        const uint32_t f = (ax[j] >= 0 && ra[j] < pv[j]) ? 1u : 0u;  // r_a below the pivot rank
         c += (uint32_t)__popcll(__ballot(f));
     }
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = c;
@@ -853,7 +859,7 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
         hipLaunchKernelGGL(k_make_keys32, dim3(nb(N, 256)), dim3(256), 0, st, pos, N,
                            c.d_pavg.as<double>(), k32, vals, seg, need, flag);
         FMM_LAUNCH_CHECK();
-        const bool alt = radix_sort32(k32, k32a, vals, vals_alt, N, 3, 0, 32,
+        const bool alt = radix_sort32(k32, k32a, vals, vals_alt, N, 3, 0, PRESORT_BITS,
                                       c.d_rs_scratch.as<uint32_t>(), st);
         const uint32_t *ks = alt ? k32a : k32;
         uint32_t *pm = alt ? vals_alt : vals;
@@ -951,7 +957,10 @@ void build_tree_device(Ctx &c, const double *pos, const double *mom, int64_t N, 
         FMM_LAUNCH_CHECK();
     }
     const int nleaf = (int)S.leaves.size();
-    hipLaunchKernelGGL(k_leaf_sort, dim3(nleaf), dim3(256), 0, st, c.d_leaves.as<int32_t>(), lo,
+    // one lane per particle of the largest leaf (a 64-particle leaf: one wave, where a
+    // 256-thread block would leave three idle)
+    const int bt_leaf = std::min(256, std::max(64, 64 * ((S.maxleaf + 63) / 64)));
+    hipLaunchKernelGGL(k_leaf_sort, dim3(nleaf), dim3(bt_leaf), 0, st, c.d_leaves.as<int32_t>(), lo,
                        cnt, rk, perm, c.d_parind.as<int32_t>(), c.d_leafof.as<int32_t>(), pos, mom,
                        c.d_sp8.as<double>(),
                        c.precision == FMM4RB_MIXED ? c.d_sp8f.as<float>() : (float *)nullptr);

@@ -159,12 +159,12 @@ int fmm4rb_field_vjp_device(fmm4rb_ctx *ctx, int64_t N, const double *d_pos, con
                             double *d_gpos, double *d_gmom, fmm4rb_stats *stats);
 /*
  * Pullback after a forward call (no reference counterpart; the pullback of the
- * rrule): the same reverse mode, with the tree and lists recomputed, but the final
- * local expansions taken from this context's last call, which must have been a
- * completed fmm4rb_field or fmm4rb_field_device (not a refit) with the same inputs
- * (checked by a 64-bit digest of pos and mom), N, N0, n, eta and precision; else
- * FMM4RB_EINVAL.
- * Skips the forward M2L and L2L that fmm4rb_field_vjp recomputes.  Results equal
+ * rrule): the same reverse mode on the tree, lists, multipoles and final local
+ * expansions of this context's last call, which must have been a completed
+ * fmm4rb_field or fmm4rb_field_device (not a refit) with the same inputs (checked by
+ * a 64-bit digest of pos and mom), N, N0, n, eta and precision; else FMM4RB_EINVAL.
+ * Skips the tree, list and upward-pass builds and the forward M2L and L2L that
+ * fmm4rb_field_vjp recomputes.  Results equal
  * fmm4rb_field_vjp's up to the rounding of the forward locals (FP64 ~1e-15,
  * mixed ~1e-7 relative).  Any other call on the context between the two (except
  * another pullback) makes it fail.

@@ -1,6 +1,6 @@
 """Pullback after a forward call (fmm4rb_field_pullback / _device): the reverse mode
-with the tree and lists recomputed and the final local expansions of the preceding
-fmm4rb_field call reused.  Pinned to fmm4rb_field_vjp (itself pinned to finite
+on the tree, lists, multipoles and final local expansions of the preceding
+fmm4rb_field call (a second pullback of the same call must give the same bits).  Pinned to fmm4rb_field_vjp (itself pinned to finite
 differences, the oracle and the BruteForce adjoint): equal up to the rounding of the
 forward locals; at eta = 0 (no far field) bit-identical; and it refuses to run on
 anything but the inputs and parameters of the last completed forward call."""
