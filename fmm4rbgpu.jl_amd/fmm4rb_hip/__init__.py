@@ -258,8 +258,8 @@ class Context:
 
     def field_pullback(self, pos, mom, Ebar, Bbar, charge=-1.0, lam=1.0, n=4, N0=125, eta=0.5,
                        precision="fp64"):
-        """field_vjp() after a field() call on the same inputs and parameters, reusing
-        its final local expansions (the tree and lists are recomputed)."""
+        """field_vjp() after a field() call on the same inputs and parameters, on that
+        call's tree, lists, multipoles and final local expansions."""
         pos, mom = _f64(pos), _f64(mom)
         N = pos.shape[0]
         gx, gp = np.empty((N, 3)), np.empty((N, 3))

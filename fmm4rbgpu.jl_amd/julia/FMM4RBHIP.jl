@@ -108,9 +108,10 @@ end
 Reverse mode of `fields` at fixed tree and interaction lists (rebuilt, not stored):
 gpos = sum Ebar.dE/dpos + Bbar.dB/dpos, gmom likewise, in `alg.precision` (the
 mixed-precision gradients agree with FP64 to ~2e-7 relative).  `after_forward = true`
-(the rrule's pullback) first tries `fmm4rb_field_pullback`, which reuses the local
-expansions of the context's last forward call when its inputs and parameters match
-(about 7% cheaper), and falls back to the self-contained `fmm4rb_field_vjp`.
+(the rrule's pullback) first tries `fmm4rb_field_pullback`, which reuses the tree,
+lists, multipoles and local expansions of the context's last forward call when its
+inputs and parameters match (about 8% cheaper), and falls back to the self-contained
+`fmm4rb_field_vjp`.
 """
 function field_vjp(positions::AbstractVector, momenta::AbstractVector, Ebar::AbstractVector,
                    Bbar::AbstractVector, charge::Real, alg::FMMHIP; lambda,
