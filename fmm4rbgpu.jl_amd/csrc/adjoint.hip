@@ -1084,35 +1084,37 @@ __device__ inline T pair_geo(double R0d, double R1d, double R2d, T px, T py, T p
 // own leaf, and the self pair (source k == selfk + 16 q) is skipped.
 // NT targets per lane, 64 / NT lanes per group; the first Q of them (the last wave of
 // a leaf may have fewer targets than 16 NT).
-template <bool DIAG, int NT, int Q, typename T, typename V4>
-__device__ __forceinline__ void adj_t_tile(const double4 *sx, const V4 *sq, int g, int m,
-                                           int selfk, const double (&xi)[NT][3],
-                                           const T (&e)[NT][3], const T (&b)[NT][3],
-                                           T (&part)[NT][3]) {
-    for (int k = g; k < m; k += NT) {
-        const double4 xs = sx[k];
-        const V4 sqk = sq[k];  // gamma, px, py, pz
+// m: a multiple of NT (the tile is padded), the same trip count in every lane.
+template <bool DIAG, int NT, int Q, typename T, typename Src>
+__device__ __forceinline__ void adj_t_tile(const Src *ts, int g, int m, int selfk,
+                                           const double (&xi)[NT][3], const T (&e)[NT][3],
+                                           const T (&b)[NT][3], T (&part)[NT][3]) {
+    for (int it = 0; it < m / NT; ++it) {
+        const int k = NT * it + g;
+        const Src sr = ts[k];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             T R[3], pr;
             const bool self = DIAG && k == selfk + (64 / NT) * q;
-            const T y = pair_geo<T>(xi[q][0] - xs.x, xi[q][1] - xs.y, xi[q][2] - xs.z, sqk.y,
-                                    sqk.z, sqk.w, self, R, pr);
+            const T y = pair_geo<T>(xi[q][0] - sr.x, xi[q][1] - sr.y, xi[q][2] - sr.z, sr.px,
+                                    sr.py, sr.pz, self, R, pr);
             const T y2 = y * y, y3 = y2 * y;
-            const T u0 = fma(sqk.x, e[q][0], fma(b[q][1], sqk.w, -(b[q][2] * sqk.z)));
-            const T u1 = fma(sqk.x, e[q][1], fma(b[q][2], sqk.y, -(b[q][0] * sqk.w)));
-            const T u2 = fma(sqk.x, e[q][2], fma(b[q][0], sqk.z, -(b[q][1] * sqk.y)));
+            const T u0 = fma(sr.g, e[q][0], fma(b[q][1], sr.pz, -(b[q][2] * sr.py)));
+            const T u1 = fma(sr.g, e[q][1], fma(b[q][2], sr.px, -(b[q][0] * sr.pz)));
+            const T u2 = fma(sr.g, e[q][2], fma(b[q][0], sr.py, -(b[q][1] * sr.px)));
             const T Ru = fma(R[0], u0, fma(R[1], u1, R[2] * u2));
             const T c3 = (T)3 * (y3 * (y2 * Ru));
-            part[q][0] = fma(y3, u0, fma(-c3, fma(pr, sqk.y, R[0]), part[q][0]));
-            part[q][1] = fma(y3, u1, fma(-c3, fma(pr, sqk.z, R[1]), part[q][1]));
-            part[q][2] = fma(y3, u2, fma(-c3, fma(pr, sqk.w, R[2]), part[q][2]));
+            part[q][0] = fma(y3, u0, fma(-c3, fma(pr, sr.px, R[0]), part[q][0]));
+            part[q][1] = fma(y3, u1, fma(-c3, fma(pr, sr.py, R[1]), part[q][1]));
+            part[q][2] = fma(y3, u2, fma(-c3, fma(pr, sr.pz, R[2]), part[q][2]));
         }
     }
 }
 
+// The mixed form is held to 128 VGPRs (4 waves per SIMD; it needs 130 unbounded).
 template <bool MIXED>
-__global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ leaves,
+__global__ __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(MIXED ? 4 : 1)))
+void k_adj_p2p_t(const int32_t *__restrict__ leaves,
                                                   const int32_t *__restrict__ off,
                                                   const uint64_t *__restrict__ keys, int sb,
                                                   const int32_t *__restrict__ lo,
@@ -1126,10 +1128,10 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ le
     // 118 instead of 220 VGPRs, 4 waves per SIMD instead of 2) ran slower, 22.8 ->
     // 24.0 ms at N = 2^22: twice the LDS reads per pair.
     constexpr int NT = PNG, TL = 64 / NT;
-    extern __shared__ double4 adj_lds[];  // sized by the launch: TC entries each
+    using Src = P2PSrc<MIXED>;
+    extern __shared__ double4 adj_lds[];  // sized by the launch: TC records
     const int BT = blockDim.x, tid = threadIdx.x;
-    double4 *sx = adj_lds;
-    V4 *sq = reinterpret_cast<V4 *>(adj_lds + TC);
+    Src *ts = reinterpret_cast<Src *>(adj_lds);
     const int lane = tid & 63, li = lane % TL, g = lane / TL, wbase = tid - lane;
     const int32_t t = leaves[xcd_item(blockIdx.x, gridDim.x)];
     const int32_t a = lo[t], ct = cnt[t];
@@ -1161,8 +1163,7 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ le
             for (int j = tid; j < take; j += BT) {
                 const double4 x4 = p8[2 * (src0 + j)];
                 const double4 p4 = p8[2 * (src0 + j) + 1];
-                sx[slot0 + j] = x4;
-                sq[slot0 + j] = V4{(T)x4.w, (T)p4.x, (T)p4.y, (T)p4.z};
+                ts[slot0 + j] = Src{x4.x, x4.y, x4.z, (T)x4.w, (T)p4.x, (T)p4.y, (T)p4.z};
             }
         };
         // each tile: staged, then processed at this one call site
@@ -1190,6 +1191,9 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ le
                 if (self) break;
             }
             if (fill == 0) break;
+            // padded to a multiple of NT (TC is one) with sources that add zero
+            const int fillp = (fill + NT - 1) / NT * NT;
+            if (tid < fillp - fill) ts[fill + tid] = Src::pad();
             __syncthreads();
             T part[NT][3];
 #pragma unroll
@@ -1197,15 +1201,15 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_t(const int32_t *__restrict__ le
             // targets of this wave in this pass: 16 nq slots (wave-uniform)
             const int nq = max(0, min(NT, (ct - tb - wbase + TL - 1) / TL));
             if (self) {
-                if (nq == NT) adj_t_tile<true, NT, NT>(sx, sq, g, fill, selfk, xi, e, b, part);
-                else if (nq == 3) adj_t_tile<true, NT, 3>(sx, sq, g, fill, selfk, xi, e, b, part);
-                else if (nq == 2) adj_t_tile<true, NT, 2>(sx, sq, g, fill, selfk, xi, e, b, part);
-                else if (nq == 1) adj_t_tile<true, NT, 1>(sx, sq, g, fill, selfk, xi, e, b, part);
+                if (nq == NT) adj_t_tile<true, NT, NT>(ts, g, fillp, selfk, xi, e, b, part);
+                else if (nq == 3) adj_t_tile<true, NT, 3>(ts, g, fillp, selfk, xi, e, b, part);
+                else if (nq == 2) adj_t_tile<true, NT, 2>(ts, g, fillp, selfk, xi, e, b, part);
+                else if (nq == 1) adj_t_tile<true, NT, 1>(ts, g, fillp, selfk, xi, e, b, part);
             } else {
-                if (nq == NT) adj_t_tile<false, NT, NT>(sx, sq, g, fill, 0, xi, e, b, part);
-                else if (nq == 3) adj_t_tile<false, NT, 3>(sx, sq, g, fill, 0, xi, e, b, part);
-                else if (nq == 2) adj_t_tile<false, NT, 2>(sx, sq, g, fill, 0, xi, e, b, part);
-                else if (nq == 1) adj_t_tile<false, NT, 1>(sx, sq, g, fill, 0, xi, e, b, part);
+                if (nq == NT) adj_t_tile<false, NT, NT>(ts, g, fillp, 0, xi, e, b, part);
+                else if (nq == 3) adj_t_tile<false, NT, 3>(ts, g, fillp, 0, xi, e, b, part);
+                else if (nq == 2) adj_t_tile<false, NT, 2>(ts, g, fillp, 0, xi, e, b, part);
+                else if (nq == 1) adj_t_tile<false, NT, 1>(ts, g, fillp, 0, xi, e, b, part);
             }
 #pragma unroll
             for (int q = 0; q < NT; ++q)
@@ -1260,10 +1264,10 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
     // VGPRs, 2 waves per SIMD).  FP64 with one source per lane (94 VGPRs, 5 waves)
     // ran slower, 30.8 -> 33.4 ms at N = 2^22: twice the LDS reads per pair.
     constexpr int NG = 2, TL = 64 / NG;
-    extern __shared__ double4 adj_lds[];  // sized by the launch: blockDim.x entries each
+    using Tgt = AdjTgt<MIXED>;
+    extern __shared__ double4 adj_lds[];  // sized by the launch: blockDim.x records
     const int BT = blockDim.x, tid = threadIdx.x;
-    double4 *tx = adj_lds;
-    V4 *te = reinterpret_cast<V4 *>(adj_lds + BT), *tb4 = te + BT;
+    Tgt *tt = reinterpret_cast<Tgt *>(adj_lds);
     const int lane = tid & 63, li = lane % TL, g = lane / TL, wbase = tid - lane;
     const int32_t s = leaves[xcd_item(blockIdx.x, gridDim.x)];
     const int32_t a = lo[s], cs = cnt[s];
@@ -1302,12 +1306,16 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
                 const int i = t0 + tid;
                 if (i < ct) {
                     const int64_t ii = ta + i;
-                    tx[tid] = p8[2 * ii];
-                    te[tid] = V4{(T)ob[6 * ii], (T)ob[6 * ii + 1], (T)ob[6 * ii + 2], (T)0};
-                    tb4[tid] = V4{(T)ob[6 * ii + 3], (T)ob[6 * ii + 4], (T)ob[6 * ii + 5], (T)0};
+                    const double4 x4 = p8[2 * ii];
+                    tt[tid] = Tgt{x4.x, x4.y, x4.z, (T)ob[6 * ii], (T)ob[6 * ii + 1],
+                                  (T)ob[6 * ii + 2], (T)ob[6 * ii + 3], (T)ob[6 * ii + 4],
+                                  (T)ob[6 * ii + 5]};
+                } else if (i == ct) {
+                    tt[tid] = Tgt::pad();  // an odd tile padded to a multiple of NG
                 }
                 __syncthreads();
                 const int m = min(BT, ct - t0);
+                const int mp = (m + NG - 1) / NG * NG;  // <= BT (a multiple of 64)
                 T part[NG][NA];
                 if constexpr (MIXED)
 #pragma unroll
@@ -1319,33 +1327,34 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
                 // keeps one copy, which needs fewer registers)
                 const int selfk = s == t ? sb0 + wbase + li - t0 : -(1 << 30);
                 auto tile = [&](auto diag) {
-                for (int k = g; k < m; k += NG) {
-                    const double4 xi = tx[k];
-                    const V4 e = te[k], bb = tb4[k];
+                for (int it = 0; it < mp / NG; ++it) {  // the same trip count in every lane
+                    const int k = NG * it + g;
+                    const Tgt tr = tt[k];
+                    const T ex = tr.e0, ey = tr.e1, ez = tr.e2, bx = tr.b0, by = tr.b1, bz = tr.b2;
 #pragma unroll
                     for (int q = 0; q < NG; ++q) {
                         const T px = ps[q][0], py = ps[q][1], pz = ps[q][2];
                         T R[3], pr;
                         const bool self = decltype(diag)::value && k == selfk + TL * q;
-                        const T y = pair_geo<T>(xi.x - xs[q][0], xi.y - xs[q][1], xi.z - xs[q][2],
+                        const T y = pair_geo<T>(tr.x - xs[q][0], tr.y - xs[q][1], tr.z - xs[q][2],
                                                 px, py, pz, self, R, pr);
                         const T y2 = y * y, y3 = y2 * y;
-                        const T s0 = fma(e.x, R[0], fma(e.y, R[1], e.z * R[2]));
-                        const T v0 = fma(R[1], bb.z, -(R[2] * bb.y));
-                        const T v1 = fma(R[2], bb.x, -(R[0] * bb.z));
-                        const T v2 = fma(R[0], bb.y, -(R[1] * bb.x));
+                        const T s0 = fma(ex, R[0], fma(ey, R[1], ez * R[2]));
+                        const T v0 = fma(R[1], bz, -(R[2] * by));
+                        const T v1 = fma(R[2], bx, -(R[0] * bz));
+                        const T v2 = fma(R[0], by, -(R[1] * bx));
                         const T Ru = fma(gs[q], s0, fma(px, v0, fma(py, v1, pz * v2)));
                         const T c = y3 * (y2 * Ru);
                         const T cp = c * pr;
                         T *P;
                         if constexpr (MIXED) P = part[q];
                         else P = acc[q];  // FP64: the sums themselves
-                        P[0] = fma(y3, e.x, P[0]);
-                        P[1] = fma(y3, e.y, P[1]);
-                        P[2] = fma(y3, e.z, P[2]);
-                        P[3] = fma(y3, bb.x, P[3]);
-                        P[4] = fma(y3, bb.y, P[4]);
-                        P[5] = fma(y3, bb.z, P[5]);
+                        P[0] = fma(y3, ex, P[0]);
+                        P[1] = fma(y3, ey, P[1]);
+                        P[2] = fma(y3, ez, P[2]);
+                        P[3] = fma(y3, bx, P[3]);
+                        P[4] = fma(y3, by, P[4]);
+                        P[5] = fma(y3, bz, P[5]);
                         P[6] = fma(c, R[0], P[6]);
                         P[7] = fma(c, R[1], P[7]);
                         P[8] = fma(c, R[2], P[8]);
@@ -1596,8 +1605,8 @@ void run_adjoint(Ctx &c, const double *Ebar, const double *Bbar, double amp, dou
         FMM_HIP(hipEventRecord(c.evx[1], sa));
         int bt = ((S.maxleaf + 63) / 64) * 64;
         bt = std::max(64, std::min(bt, PT));
-        const size_t v4 = mixed ? sizeof(float4) : sizeof(double4);
-        const size_t per_t = sizeof(double4) + v4, per_s = sizeof(double4) + 2 * v4;
+        const size_t per_t = mixed ? sizeof(P2PSrc<true>) : sizeof(P2PSrc<false>);
+        const size_t per_s = mixed ? sizeof(AdjTgt<true>) : sizeof(AdjTgt<false>);
         // target side: tiles of up to 4 blocks of sources, within 12 KB of LDS per wave
         // (so that LDS does not cap the 4 waves per SIMD the registers allow)
         auto tcap = [&](size_t per) {

@@ -15,6 +15,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "fmm4rb.h"
@@ -81,6 +82,26 @@ __device__ inline T rsq3(T d) {
     else y = rsqrt_nr(d);
     return (y * y) * y;
 }
+// Particle records staged in LDS by the near-field kernels (forward P2P and both P2P^T
+// sides): the position in FP64, then the per-particle terms (FP32 in the mixed
+// kernels), one record per particle so that one address register walks a tile.  A
+// tile is padded to the kernel's step with records at 1e30 whose terms are zero: they
+// add exactly zero (R.R overflows to inf in FP32, so d^-3/2 = 0; in FP64 every
+// product carries a zero factor).
+template <bool MIXED>
+struct P2PSrc {  // a source: gamma and p
+    using V = typename std::conditional<MIXED, float, double>::type;
+    double x, y, z;
+    V g, px, py, pz;
+    __device__ static P2PSrc pad() { return P2PSrc{1e30, 1e30, 1e30, 0, 0, 0, 0}; }
+};
+template <bool MIXED>
+struct AdjTgt {  // a target of the P2P^T source side: the cotangents e = Ebar, b = Bbar
+    using V = typename std::conditional<MIXED, float, double>::type;
+    double x, y, z;
+    V e0, e1, e2, b0, b1, b2;
+    __device__ static AdjTgt pad() { return AdjTgt{1e30, 1e30, 1e30, 0, 0, 0, 0, 0, 0}; }
+};
 constexpr int MAX_N0 = 1024;
 constexpr int TCNT_WORDS = 64, TCNT_FLAG = 63;  // 2 maxlevel + 4 <= 62 counters, then the flag
 

@@ -60,15 +60,13 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
     const int32_t *__restrict__ cnt, const double *__restrict__ sp8, const float *__restrict__ sp4f,
     double *__restrict__ nearf, int TC) {
     using T = typename std::conditional<MIXED, float, double>::type;
-    using V4 = typename std::conditional<MIXED, float4, double4>::type;
-    // LDS sized by the launch (TC entries each, a multiple of blockDim.x), not by
+    using Src = P2PSrc<MIXED>;
+    // LDS sized by the launch (TC records, a multiple of blockDim.x), not by
     // P2P_MAXT: a one-wave block holds 10 KB (mixed, 256 sources), so LDS allows 4
     // waves per SIMD, as the registers do
     extern __shared__ double2 p2p_lds[];
-    const int BT = blockDim.x;                                     // TC: tile capacity
-    double2 *txy = p2p_lds;                                        // source x, y (FP64)
-    double *tz = reinterpret_cast<double *>(p2p_lds + TC);         // source z
-    V4 *tq = reinterpret_cast<V4 *>(p2p_lds + TC + (TC + 1) / 2);  // gamma, px, py, pz
+    const int BT = blockDim.x;  // TC: tile capacity
+    Src *ts = reinterpret_cast<Src *>(p2p_lds);
     const int tid = threadIdx.x;
     const int lane = tid & 63, ti = lane & 15, g = lane >> 4;
     const int tw = (tid >> 6) * 16 * TPL;  // first target of this wave within a pass
@@ -112,13 +110,13 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
                 for (int q = 0; q < Q; ++q)
 #pragma unroll
                     for (int k = 0; k < 6; ++k) part[q][k] = 0;
-                for (int jj = g; jj < m; jj += 4) {
-                    const double2 xy = txy[jj];
-                    const double zs = tz[jj];
-                    const V4 sq = tq[jj];  // gamma, px, py, pz
+                // m is a multiple of 4 (the last tile is padded): the same trip count in
+                // every lane, one address increment per step
+                for (int it = 0; it < (m >> 2); ++it) {
+                    const Src sr = ts[4 * it + g];
 #pragma unroll
                     for (int q = 0; q < Q; ++q)
-                        p2p_pair<T>(xi[q], xy.x, xy.y, zs, sq.x, sq.y, sq.z, sq.w, part[q]);
+                        p2p_pair<T>(xi[q], sr.x, sr.y, sr.z, sr.g, sr.px, sr.py, sr.pz, part[q]);
                 }
 #pragma unroll
                 for (int q = 0; q < Q; ++q)
@@ -130,13 +128,11 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
                         hi[q][k] = t;
                     }
             } else {
-                for (int jj = g; jj < m; jj += 4) {
-                    const double2 xy = txy[jj];
-                    const double zs = tz[jj];
-                    const V4 sq = tq[jj];
+                for (int it = 0; it < (m >> 2); ++it) {
+                    const Src sr = ts[4 * it + g];
 #pragma unroll
                     for (int q = 0; q < Q; ++q)
-                        p2p_pair<T>(xi[q], xy.x, xy.y, zs, sq.x, sq.y, sq.z, sq.w, acc[q]);
+                        p2p_pair<T>(xi[q], sr.x, sr.y, sr.z, sr.g, sr.px, sr.py, sr.pz, acc[q]);
                 }
             }
         };
@@ -165,16 +161,25 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
 #pragma unroll 1
                 for (int j = tid; j < take; j += BT) {
                     const int64_t src = sa + s0 + j;
-                    const int slot = fill + j;
                     const double4 x4 = p8[2 * src];
-                    txy[slot] = make_double2(x4.x, x4.y);
-                    tz[slot] = x4.z;
+                    Src sr;
+                    sr.x = x4.x;
+                    sr.y = x4.y;
+                    sr.z = x4.z;
                     if constexpr (MIXED) {
-                        tq[slot] = q4[src];
+                        const float4 q = q4[src];  // gamma, px, py, pz
+                        sr.g = q.x;
+                        sr.px = q.y;
+                        sr.py = q.z;
+                        sr.pz = q.w;
                     } else {
                         const double4 p4 = p8[2 * src + 1];  // px, py, pz, 0
-                        tq[slot] = make_double4(x4.w, p4.x, p4.y, p4.z);
+                        sr.g = x4.w;
+                        sr.px = p4.x;
+                        sr.py = p4.y;
+                        sr.pz = p4.z;
                     }
+                    ts[fill + j] = sr;
                 }
                 fill += take;
                 s0 += take;
@@ -187,8 +192,11 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
             }
         }
         if (fill > 0) {
+            // pad to a multiple of 4 with sources that add exactly zero
+            const int fill4 = (fill + 3) & ~3;  // <= TC, a multiple of 64
+            if (tid < fill4 - fill) ts[fill + tid] = Src::pad();
             __syncthreads();
-            run_tile(fill);
+            run_tile(fill4);
         }
         // Sum the 4 source groups (lanes ti + 16 g), fixed order, in FP64.
         double tot[TPL][6];
@@ -235,15 +243,14 @@ void launch_p2p(Ctx &c, const int32_t *off, const uint64_t *keys, int sb, hipStr
     const int tpl = (!mixed && c.shape.maxleaf > 64) ? 8 : 4;
     int bt = 64 * ((c.shape.maxleaf + 16 * tpl - 1) / (16 * tpl));
     bt = std::max(64, std::min(bt, P2P_MAXT));
-    // per source: x, y (16 B), z (8 B), gamma and p (16 B mixed, 32 B FP64): 40 B read per
-    // source and lane in mixed precision instead of 48 with a padded double4
-    const size_t per = sizeof(double2) + sizeof(double) + (mixed ? sizeof(float4) : sizeof(double4));
+    // per source one record: x, y, z (24 B), gamma and p (16 B mixed, 32 B FP64): 40 B
+    // read per source and lane in mixed precision instead of 48 with a padded double4
+    const size_t per = mixed ? sizeof(P2PSrc<true>) : sizeof(P2PSrc<false>);
     int tc = bt;  // a multiple of the block size, >= 1 block
     while (tc + bt <= P2P_TILE_MUL * bt &&
            (size_t)(tc + bt) * per + 16 <= P2P_TILE_LDS_WAVE * (size_t)(bt / 64))
         tc += bt;
-    const size_t lds = (size_t)tc * sizeof(double2) + (size_t)((tc + 1) / 2) * sizeof(double2) +
-                       (size_t)tc * (mixed ? sizeof(float4) : sizeof(double4));
+    const size_t lds = (size_t)tc * per;
 #define P2P_L(MX, TP)                                                                            \
     hipLaunchKernelGGL((k_p2p<MX, TP>), dim3(nleaf), dim3(bt), lds, st,                          \
                        c.d_leaves.as<int32_t>(), off, keys, sb, c.d_lo.as<int32_t>(),             \
