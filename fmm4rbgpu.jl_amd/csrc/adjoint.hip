@@ -1123,7 +1123,6 @@ void k_adj_p2p_t(const int32_t *__restrict__ leaves,
                                                   const double *__restrict__ ob,
                                                   double *__restrict__ gx, int TC) {
     using T = typename std::conditional<MIXED, float, double>::type;
-    using V4 = typename std::conditional<MIXED, float4, double4>::type;
     // targets per lane: 4 (16 lanes x 4 groups).  FP64 with 2 (32 lanes x 2 groups,
     // 118 instead of 220 VGPRs, 4 waves per SIMD instead of 2) ran slower, 22.8 ->
     // 24.0 ms at N = 2^22: twice the LDS reads per pair.
@@ -1255,7 +1254,6 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
                                                   const double *__restrict__ ob,
                                                   double *__restrict__ gx, double *__restrict__ gp) {
     using T = typename std::conditional<MIXED, float, double>::type;
-    using V4 = typename std::conditional<MIXED, float4, double4>::type;
     // sums: E3 [0-2], B3 [3-5], sum c R [6-8], sum c pr [9], sum y3 e.R [10],
     //       sum y3 R x b [11-13], sum c pr R [14-16]
     constexpr int NA = 17;
