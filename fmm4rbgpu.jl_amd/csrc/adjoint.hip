@@ -612,7 +612,7 @@ __attribute__((amdgpu_waves_per_eu(AdjTWaves<NP, MIXED>::v))) void k_adj_m2l_t(
     double(*pr3)[3] = reinterpret_cast<double(*)[3]>(shb);
     const V4 *mp = reinterpret_cast<const V4 *>(mpv);
     const int tid = threadIdx.x;
-    const int32_t t = xcd_item(blockIdx.x, gridDim.x);
+    const int32_t t = xcd_item<XCD_RUN_FAR>(blockIdx.x, gridDim.x);
     const int32_t beg = off[t], end = off[t + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
     const T px = (T)pavg4[0], py = (T)pavg4[1], pz = (T)pavg4[2];
@@ -893,7 +893,7 @@ __global__ __launch_bounds__(AShape<NP>::BLK) void k_adj_m2l_s(
     const V4 *mp = reinterpret_cast<const V4 *>(mpv);
     const double *lbar = reinterpret_cast<const double *>(lbarv);
     const int tid = threadIdx.x;
-    const int32_t s = xcd_item(blockIdx.x, gridDim.x);
+    const int32_t s = xcd_item<XCD_RUN_FAR>(blockIdx.x, gridDim.x);
     const int32_t beg = offT[s], end = offT[s + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
     const T px = (T)pavg4[0], py = (T)pavg4[1], pz = (T)pavg4[2];
@@ -1132,7 +1132,7 @@ void k_adj_p2p_t(const int32_t *__restrict__ leaves,
     const int BT = blockDim.x, tid = threadIdx.x;
     Src *ts = reinterpret_cast<Src *>(adj_lds);
     const int lane = tid & 63, li = lane % TL, g = lane / TL, wbase = tid - lane;
-    const int32_t t = leaves[xcd_item(blockIdx.x, gridDim.x)];
+    const int32_t t = leaves[xcd_item<XCD_RUN_NEAR>(blockIdx.x, gridDim.x)];
     const int32_t a = lo[t], ct = cnt[t];
     const int32_t beg = off[t], end = off[t + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
@@ -1267,7 +1267,7 @@ __global__ __launch_bounds__(PT) void k_adj_p2p_s(const int32_t *__restrict__ le
     const int BT = blockDim.x, tid = threadIdx.x;
     Tgt *tt = reinterpret_cast<Tgt *>(adj_lds);
     const int lane = tid & 63, li = lane % TL, g = lane / TL, wbase = tid - lane;
-    const int32_t s = leaves[xcd_item(blockIdx.x, gridDim.x)];
+    const int32_t s = leaves[xcd_item<XCD_RUN_NEAR>(blockIdx.x, gridDim.x)];
     const int32_t a = lo[s], cs = cnt[s];
     const int32_t beg = offT[s], end = offT[s + 1];
     const uint64_t mask = (1ull << sb) - 1ull;

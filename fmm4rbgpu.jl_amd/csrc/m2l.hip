@@ -177,7 +177,7 @@ __attribute__((amdgpu_waves_per_eu(m2l_waves<NP, MIXED>()))) void k_m2l(
     const int sk0 = S > 1 ? (part == 0 ? 0 : (NP + 1) / 2) : 0;
     const int sk1 = S > 1 ? (part == 0 ? (NP + 1) / 2 : NP) : NP;
     const int lnode = S > 1 ? tid - part * NN : tid;  // this lane's node (S > 1)
-    const int32_t t = xcd_item(blockIdx.x, gridDim.x);
+    const int32_t t = xcd_item<XCD_RUN_FAR>(blockIdx.x, gridDim.x);
     const int32_t beg = off[t], end = off[t + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
     const T px = (T)pavg4[0], py = (T)pavg4[1], pz = (T)pavg4[2];

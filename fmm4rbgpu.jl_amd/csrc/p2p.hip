@@ -71,7 +71,7 @@ __global__ __launch_bounds__(P2P_MAXT) void k_p2p(
     const int lane = tid & 63, ti = lane & 15, g = lane >> 4;
     const int tw = (tid >> 6) * 16 * TPL;  // first target of this wave within a pass
     const int per_pass = (BT >> 6) * 16 * TPL;
-    const int32_t t = leaves[xcd_item(blockIdx.x, gridDim.x)];
+    const int32_t t = leaves[xcd_item<XCD_RUN_NEAR>(blockIdx.x, gridDim.x)];
     const int32_t a = lo[t], ct = cnt[t];
     const int32_t beg = off[t], end = off[t + 1];
     const uint64_t mask = (1ull << sb) - 1ull;
