@@ -581,15 +581,20 @@ struct YSplit {
 // Occupancy: the FP64 form with the source-node sums (4 <= NP <= 6) is held to 3 waves
 // per SIMD (168 VGPRs, no spills): 167.5 -> 162.6 ms at N = 2^22, n = 4.  At NP = 7 the
 // same bound made it 15% slower (rematerialisation), so larger NP keep the default (NP = 7
-// is held to 3 waves per SIMD by its 78 KB of LDS anyway).
-template <int NP, bool MIXED>
+// is held to 3 waves per SIMD by its 78 KB of LDS anyway).  The mixed form at NP = 5
+// without the forward sums (the pullback) is held to 4 waves per SIMD (128 instead of 152
+// VGPRs, 16 bytes spilled outside the node loop; its 19 KB of LDS allow 4): 31.2 -> 29.5
+// ms alone at N = 2^22, mixed pullback 82.9 -> 81.0 ms; with the forward sums the same
+// bound spilled 76 bytes and gained nothing.
+template <int NP, bool MIXED, bool FWD>
 struct AdjTWaves {
-    static constexpr int v = (!MIXED && NP >= 4 && NP <= 6) ? 3 : 1;
+    static constexpr int v =
+        (!MIXED && NP >= 4 && NP <= 6) ? 3 : ((MIXED && !FWD && NP == 5) ? 4 : 1);
 };
 
 template <int NP, bool FWD, bool MIXED, bool QUAD = false>
 __global__ __launch_bounds__(AShape<NP>::BLK)
-__attribute__((amdgpu_waves_per_eu(AdjTWaves<NP, MIXED>::v))) void k_adj_m2l_t(
+__attribute__((amdgpu_waves_per_eu(AdjTWaves<NP, MIXED, FWD>::v))) void k_adj_m2l_t(
     const int32_t *__restrict__ off, const uint64_t *__restrict__ keys, int sb,
     const double *__restrict__ ncoord, const void *__restrict__ mpv,
     const double *__restrict__ lbar, const double *__restrict__ pavg4, double *__restrict__ Xbar,
