@@ -149,9 +149,12 @@ __device__ inline void m2l_pair(double tx, double ty, double tz, const double *s
 // gfx950 issues two independent FP32 VALU instructions per cycle (two-operand
 // encodings, VGPR operands, from different waves), so more resident waves pair more
 // of them: 16.29 -> 15.54 ms alone at N = 2^22 with the table form of d (8 waves
-// spilled and took 17.6 ms).
+// spilled and took 17.6 ms).  NP = 7 likewise (128 -> 80 VGPRs, 92 bytes spilled outside
+// the inner loop): its 11-wave blocks fit twice per CU at 6 waves per SIMD and once at 4
+// or 5, 136 -> 124 ms alone at N = 2^22, n = 6.  NP = 6 ran 3% slower held to 6 waves, and
+// the FP64 kernel at NP = 7 held to 5 waves 12% slower.
 template <int NP, bool MIXED>
-constexpr int m2l_waves() { return MIXED && NP == 5 ? 6 : 1; }  // (n = 3: 39 VGPRs spilled)
+constexpr int m2l_waves() { return MIXED && (NP == 5 || NP == 7) ? 6 : 1; }  // (n = 3: 39 VGPRs spilled)
 
 template <int NP, bool MIXED, bool QUAD>
 __global__ __launch_bounds__((M2LShape<NP, MIXED>::BLK))
