@@ -1,8 +1,8 @@
 """The Julia binding's ccalls against the C ABI (no Julia needed).
 
 Every `ccall((:fmm4rb_x, libfmm4rb), Ret, (Types...), args...)` in
-fmm4rbgpu.jl_amd/julia/FMM4RBHIP.jl and in INTEGRATION.md's Julia snippet must
-name a function that include/fmm4rb.h declares, with the Julia types of the
+fmm4rbgpu.jl_amd/julia/FMM4RBHIP.jl, and in the Julia code blocks of the integration
+guide at the repository root, must name a function that include/fmm4rb.h declares, with the Julia types of the
 header's return and parameter types, and pass as many arguments as it declares.
 Also checks that the binding registers its update_particles_field! method by the
 reference package's UUID (any load order), not by a name lookup in Main."""
