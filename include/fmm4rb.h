@@ -122,7 +122,7 @@ int fmm4rb_field_device_noalloc(fmm4rb_ctx *ctx, int64_t N, const double *d_pos,
  * interpolation nodes and all expansions.  With the inputs of that call the
  * result is bit-identical to it; as particles drift the partition goes stale
  * and the error grows until the caller rebuilds with fmm4rb_field*.  Skips the
- * tree build and the traversal (about 7 of 34 ms at N = 2^22 mixed).
+ * tree build and the traversal (about 3 of 26.6 ms at N = 2^22 mixed).
  */
 int fmm4rb_field_refit(fmm4rb_ctx *ctx, int64_t N, const double *pos, const double *mom,
                        double charge, double lambda, int n, int64_t N0, double eta,
