@@ -21,9 +21,9 @@ METRIC = "particles/s field eval N=2^22: mixed-prec vs FP64 HIP FMM at rel err <
 PEAK = {"mixed": 157.3e12, "fp64": 78.6e12}  # MI355X FP32 / FP64 vector peaks (FLOP/s)
 FLOPS_PER_PAIR = 40  # one kernel evaluation, P2P particle pair or M2L node pair
 # HBM bytes per launch from two rocprofv3 PMC passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000,
-# calibrated on a 1 GiB copy), N=2^22, n=4: profiles/m2l_p2p_bunch_2e22_n4_valu.md
-TRAFFIC = {(1 << 22, 4, "mixed", "m2l"): 4.307e9, (1 << 22, 4, "mixed", "p2p"): 5.922e9,
-           (1 << 22, 4, "fp64", "m2l"): 1.539e10, (1 << 22, 4, "fp64", "p2p"): 4.044e9}
+# calibrated on a 1 GiB copy), N=2^22, n=4: profiles/m2l_p2p_bunch_2e22_n4_traffic.md
+TRAFFIC = {(1 << 22, 4, "mixed", "m2l"): 3.523e9, (1 << 22, 4, "mixed", "p2p"): 3.000e9,
+           (1 << 22, 4, "fp64", "m2l"): 1.266e10, (1 << 22, 4, "fp64", "p2p"): 2.045e9}
 
 
 def parse():
