@@ -91,8 +91,9 @@ int fmm4rb_field_device(fmm4rb_ctx *ctx, int64_t N, const double *d_pos, const d
 
 /*
  * Allocation-free variant of fmm4rb_field_device.  No reference counterpart: it
- * is for callers that cannot allocate device memory (code under HIP graph
- * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * is for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc); like every entry point it synchronizes the
+ * host (list capacities), so it cannot itself be graph-captured.  Same
  * arguments, semantics and results as fmm4rb_field_device (bit-identical), but
  * every device buffer of the call is carved from a static scratch array of
  * 256 MiB per device (each device the library runs on holds it until the
